@@ -1,0 +1,160 @@
+/*
+ * serbf.h -- device-resident engine API of the MI355X RBF-FD tracer-advection
+ * engine.  Plain C ABI: plain pointers and sizes, int status codes, no torch
+ * or HIP types.  The reference-compatible drop-in symbols (get_stencils,
+ * get_rbffd_DMs, get_solution, ...) are in serbf_ref.h and are built on top of
+ * this API.
+ *
+ * Data conventions (host side):
+ *   nodes      x, y, z double[N] on the unit sphere
+ *   stencils   idx int32[N*n], D double[N*n], row-major, sorted by (d^2, index)
+ *   weights    Dx, Dy, L double[N*n]: d/dx', d/dy' in the node's rotated frame
+ *              (unit sphere) and the Delta^k operator (unit sphere)
+ *   state      q double[N*ntracers*Nv], node-major ([node][tracer][level])
+ */
+#ifndef SERBF_H
+#define SERBF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SERBF_OK 0
+#define SERBF_ERR_ARG 1          /* invalid argument or size */
+#define SERBF_ERR_HIP 2          /* HIP runtime error (message in serbf_last_error) */
+#define SERBF_ERR_SINGULAR 3     /* at least one weight system was singular */
+#define SERBF_ERR_UNSUPPORTED 4  /* configuration outside what the kernels support */
+#define SERBF_ERR_STATE 5        /* call order: a prerequisite has not been set */
+#define SERBF_ERR_NOMEM 6
+
+/* method: how the weights and the vertical operator are built */
+#define SERBF_MODE_REFERENCE 0   /* reference solver: PHS r^k + degree-5 tangent-plane
+                                    monomials (unscaled), Gaussian+constant Delta^k system,
+                                    7-point FD1 vertical with zero ghosts, H = I at the
+                                    south pole (rotated vertical velocity sign kept) */
+#define SERBF_MODE_CONFIG 1      /* PHS r^k + degree-d tangent-plane monomials on radius-
+                                    scaled stencils, PHS Delta^k system, 1D RBF-FD vertical
+                                    stencils (one-sided at the boundaries), radial
+                                    velocity taken as w everywhere */
+
+/* flows */
+#define SERBF_FLOW_TC2 0         /* reference test case 2, Hadley-like overturning */
+#define SERBF_FLOW_SOLID_BODY 1  /* tilted solid-body rotation, w = 0 */
+#define SERBF_FLOW_DEFORM 2      /* solid-body rotation + reversing swirl + reversing w */
+
+/* precision of the time loop */
+#define SERBF_F64 0
+#define SERBF_F32 1
+
+typedef struct serbf_config {
+    /* vertical geometry: cell-centred levels h_v = dh (v + 1/2), dh = htop / Nv */
+    int Nv;
+    double htop;            /* m */
+    double R;               /* sphere radius, m */
+
+    /* horizontal weights */
+    int mode;               /* SERBF_MODE_* */
+    int n;                  /* stencil size */
+    int phs_order;          /* odd PHS order of the gradient system */
+    int poly_deg;           /* tangent-plane polynomial degree (reference mode: 5) */
+    int hv_order;           /* k of Delta^k (0: no hyperviscosity weights) */
+    int hv_phs_order;       /* config mode: odd PHS order of the Delta^k system, >= 2k+1 */
+    double hv_eps;          /* reference mode: Gaussian shape parameter epsilon */
+
+    /* vertical operator (config mode) */
+    int nv;                 /* vertical stencil size (odd, <= 9) */
+    int v_phs_order;        /* odd PHS order of the 1D system */
+    int v_poly_deg;         /* 1D polynomial degree */
+
+    /* right-hand side: dq/dt = -U . grad q + hv_coef * (Delta^k_unit q) */
+    double hv_coef;         /* 1/s; 0 disables the term */
+
+    /* flow */
+    int flow;               /* SERBF_FLOW_* */
+    double u0;              /* solid-body equatorial speed, m/s */
+    double alpha;           /* solid-body tilt, rad */
+    double T;               /* period of the reversing parts, s */
+    double k_def;           /* swirl amplitude, m/s */
+    double w0;              /* vertical velocity amplitude, m/s */
+
+    /* tracers advected together (sharing stencils and weights); 0 means 1 */
+    int ntracers;
+} serbf_config;
+
+typedef struct serbf_engine serbf_engine;
+
+/* last error message of the calling thread */
+const char *serbf_last_error(void);
+
+/* ---- node sets (host) ---- */
+/* Icosahedral node set from recursive 4-way subdivision: N = 10*4^level + 2.
+ * Each level's set is a prefix of the next; the first 12 nodes are the icosahedron
+ * vertices.  Arrays must hold N doubles. */
+int serbf_icos_count(int level);
+int serbf_icos_nodes(int level, double *x, double *y, double *z);
+/* Seeded tangential jitter: each node moves by U(0, frac*h) in a uniformly random
+ * tangent direction and is renormalised; h = sqrt(8 pi / (sqrt(3) N)). */
+int serbf_jitter_nodes(int N, double *x, double *y, double *z, double frac, unsigned long long seed);
+/* Permutation perm[new] = old that orders nodes along a space-filling curve
+ * (Hilbert order of the cube-face coordinates) for gather locality. */
+int serbf_spatial_order(int N, const double *x, const double *y, const double *z, int *perm);
+/* lambda in [0, 2pi), phi = atan2(z, sqrt(x^2 + y^2)) */
+void serbf_lonlat(int N, const double *x, const double *y, const double *z, double *lambda, double *phi);
+
+/* initial tracer fields on cell-centred levels (q[N*Nv], node-major) */
+#define SERBF_IC_TC2 0       /* reference test case 2: cos^2 layer between 2 and 5 km */
+#define SERBF_IC_BELL 1      /* cosine bell of radius R/3 at (3pi/2, 0) x Gaussian in height
+                                (centre 5 km, sigma 1 km) */
+#define SERBF_IC_HILLS 2     /* two Gaussian hills of width 0.2R, centres from the seed,
+                                x Gaussian in height (centre htop/2, width 0.2 htop) */
+#define SERBF_IC_SLOTTED 3   /* two slotted cylinders of radius R/2 at (5pi/6, 0), (7pi/6, 0) */
+#define SERBF_IC_BELLS2 4    /* two cosine bells of radius R/2 on a 0.1 background */
+#define SERBF_IC_CONST 5     /* constant 1 (mass check) */
+int serbf_initial_condition(int kind, const serbf_config *cfg, int N, const double *x, const double *y,
+                            const double *z, unsigned long long seed, double *q);
+
+/* ---- engine ---- */
+int serbf_create(serbf_engine **out, int device);
+void serbf_destroy(serbf_engine *e);
+
+int serbf_set_nodes(serbf_engine *e, int N, const double *x, const double *y, const double *z);
+/* Optional longitude/latitude of the nodes (as read from a node file); without
+ * them the flows use lambda = atan2(y, x) in [0, 2pi), phi = atan2(z, |(x, y)|). */
+int serbf_set_lonlat(serbf_engine *e, const double *lambda, const double *phi);
+/* GPU kNN (bucketed, exact (d^2, index) order). */
+int serbf_knn(serbf_engine *e, int n);
+int serbf_get_stencils(serbf_engine *e, int *idx, double *D);
+int serbf_set_stencils(serbf_engine *e, int n, const int *idx);
+
+int serbf_configure(serbf_engine *e, const serbf_config *cfg);
+/* Batched FP64 weight solves for every node (one persistent launch). */
+int serbf_weights(serbf_engine *e);
+/* status[i] = 0 ok, 1 singular gradient system, 2 singular Delta^k system */
+int serbf_get_weights(serbf_engine *e, double *Dx, double *Dy, double *L, int *status);
+int serbf_set_weights(serbf_engine *e, const double *Dx, const double *Dy, const double *L);
+/* Vertical operator table: start[Nv], W[Nv*nv] (per-level stencil start and weights). */
+int serbf_get_vertical(serbf_engine *e, int *start, double *W, int *nv);
+
+int serbf_set_state(serbf_engine *e, const double *q);
+int serbf_get_state(serbf_engine *e, int precision, double *q);
+/* One right-hand-side evaluation F(q, t) of the FP64 state (for parity tests). */
+int serbf_rhs(serbf_engine *e, double t, double *F);
+/* nsteps classic RK4 steps from time t0 in the given precision.  The FP32 run
+ * starts from the FP64 state converted once (serbf_set_state). */
+int serbf_run(serbf_engine *e, int precision, int nsteps, double dt, double t0);
+/* Same, timed with HIP events on the engine's stream; *ms = elapsed time of the
+ * nsteps steps, *stage_ms = average duration of one stage kernel launch. */
+int serbf_run_timed(serbf_engine *e, int precision, int nsteps, double dt, double t0,
+                    double *ms, double *stage_ms);
+/* Reset the FP32 state from the FP64 state. */
+int serbf_sync_f32_from_f64(serbf_engine *e);
+/* Relative norms of state(precision) against q_ref (host, N*Nv):
+ * out = {L1, L2, Linf, mass_ref, mass, min, max}; mass uses the reference's
+ * rho(h) * 4 pi r^2 dh / N cell volumes. */
+int serbf_norms(serbf_engine *e, int precision, const double *q_ref, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SERBF_H */

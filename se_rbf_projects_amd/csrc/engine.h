@@ -1,0 +1,131 @@
+// Internal declarations of libserbf (MI355X RBF-FD tracer-advection engine).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/serbf.h"
+#include "../../include/serbf_ref.h"
+
+namespace serbf {
+
+void set_error(const std::string &msg);
+
+#define SERBF_HIP(call)                                                                   \
+    do {                                                                                  \
+        hipError_t err_ = (call);                                                         \
+        if (err_ != hipSuccess) {                                                         \
+            ::serbf::set_error(std::string(#call) + ": " + hipGetErrorString(err_));     \
+            return SERBF_ERR_HIP;                                                         \
+        }                                                                                 \
+    } while (0)
+
+#define SERBF_CHECK(cond, code, msg)                                                      \
+    do {                                                                                  \
+        if (!(cond)) {                                                                    \
+            ::serbf::set_error(msg);                                                      \
+            return code;                                                                  \
+        }                                                                                 \
+    } while (0)
+
+// Largest vertical column handled by one wavefront (lane = level).
+constexpr int kMaxLevels = 64;
+// Largest vertical stencil.
+constexpr int kMaxNv = 9;
+// Largest horizontal stencil supported by the weight solver.
+constexpr int kMaxStencil = 128;
+// Per-node velocity basis record (see flows in rk4.hip).
+constexpr int kBasis = 16;
+// Scalars per stage time in the time table.
+constexpr int kTimeScalars = 4;
+
+// Device buffers of one precision of the time loop.
+template <typename T>
+struct StateBuffers {
+    T *qT = nullptr;      // state at the start of the step   [N][ntr][S]
+    T *qA = nullptr;      // stage states (ping-pong)
+    T *qB = nullptr;
+    T *F = nullptr;       // RK4 accumulator
+    T *Wx = nullptr, *Wy = nullptr, *WL = nullptr;  // [N][n]
+    T *basis = nullptr;   // [N][kBasis]
+    T *lev = nullptr;     // [4][S]: r_inv, ga, gb, unused
+    T *vW = nullptr;      // [S][kMaxNv]
+    T *tscal = nullptr;   // [steps][4 stages][kTimeScalars]
+    T *gamma = nullptr;   // per-point hyperviscosity coefficient [N][ntr][S] (closure)
+};
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    int G = 0;
+    int precision = -1;
+    const void *key = nullptr;
+    double dt = 0.0;
+    int hv = -1;
+};
+
+}  // namespace serbf
+
+struct serbf_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int N = 0;
+    int n = 0;
+    // nodes
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr;
+    double *d_lam = nullptr, *d_phi = nullptr;   // optional given longitude/latitude
+    // stencils
+    int *d_idx = nullptr;
+    double *d_D = nullptr;
+    bool have_stencils = false;
+    // configuration
+    serbf_config cfg{};
+    bool configured = false;
+    int S = 0;      // column stride = lanes per column
+    int ntr = 1;    // tracers
+    int nv = 0;     // vertical stencil size
+    int *d_vstart = nullptr;
+    int h_vstart[serbf::kMaxLevels];
+    double h_vW[serbf::kMaxLevels * serbf::kMaxNv];
+    // weights (FP64 master copy), solve status
+    double *d_Wx = nullptr, *d_Wy = nullptr, *d_WL = nullptr;
+    int *d_status = nullptr;
+    bool have_weights = false;
+    // time loop buffers
+    serbf::StateBuffers<double> s64;
+    serbf::StateBuffers<float> s32;
+    bool have_state = false;
+    double *d_q0 = nullptr;   // FP64 copy of the state given to serbf_set_state
+    double *d_wmass = nullptr;  // per-level rho(h) 4 pi r^2 dh / N
+    bool f32_ready = false;   // FP32 weights/basis converted
+    int tcap = 0;             // time table capacity in steps
+    int *d_base = nullptr;    // step counter read by the stage kernels
+    // closure coefficient mode: 0 none/constant, 1 per-point gamma
+    int hv_mode = 0;
+    serbf::GraphCache graph;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // reusable scratch
+    void *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+namespace serbf {
+int ensure_scratch(serbf_engine *e, size_t bytes);
+int alloc_stencils(serbf_engine *e, int n);
+int alloc_weights(serbf_engine *e);
+int convert_to_f32(serbf_engine *e, size_t n, const double *src, float *dst);
+// kNN (knn.hip)
+int knn_run(serbf_engine *e, int n);
+// weights (weights.hip)
+int weights_run(serbf_engine *e);
+int vertical_table(const serbf_config &cfg, int *start, double *W);
+// time loop (rk4.hip)
+int rk4_prepare(serbf_engine *e);
+void rk4_free(serbf_engine *e);
+int rk4_rhs(serbf_engine *e, double t, double *F_host);
+int rk4_run(serbf_engine *e, int precision, int nsteps, double dt, double t0, bool timed,
+            double *ms, double *stage_ms);
+int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha,
+                            const double *q_host, double *F_host);
+}  // namespace serbf

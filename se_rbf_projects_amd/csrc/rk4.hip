@@ -1,0 +1,675 @@
+// Fused RK4 stage kernel of the RBF-FD tracer-advection engine (gfx950).
+//
+// One stage of classic RK4 (wrappers.c:54-104 of the reference) is one launch:
+// it evaluates the right-hand side of eval_partSVDotGradSV (rbffd_DMs.c:331-443)
+//     F = -((Up_x gx + Up_y gy) + Up_z gz) + c * (L q)
+//     gx = r_inv[v] sum_s Dx[p,s] q[idx[p,s], v],  gy likewise,
+//     gz = sum_j W[v,j] q[p, start[v]+j]  (zero outside the column)
+// and fuses the axpys of that stage (eval_sumpatchSVpartSV / eval_sumpartSVpartSV,
+// rbffd_DMs.c:446-520) into its epilogue:
+//     stage 1: F  = f;          q_B = q_T + dt/2 f
+//     stage 2: F  = F + 2 f;    q_A = q_T + dt/2 f
+//     stage 3: F  = F + 2 f;    q_B = q_T + dt f
+//     stage 4: q_T = q_T + dt/6 (F + 1.0 f)
+//
+// Mapping: one wavefront per node column (LANES = 64) or 64/LANES columns per
+// wavefront for short columns; lane = vertical level.  The stencil indices and
+// weights of a node are wave-uniform (scalar loads); each neighbour's column is
+// one coalesced vector load of LANES contiguous values ([node][tracer][level]
+// layout, column stride LANES).  The vertical stencil is applied to the node's
+// own column (stencil entry 0) with cross-lane permutes.  The velocity is
+// evaluated in-kernel from a per-node basis and per-level tables (no velocity
+// arrays are streamed).  Work-groups are remapped so that each XCD processes a
+// contiguous range of nodes (its L2 sees the spatially local neighbour columns).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_util.h"
+#include "engine.h"
+
+namespace serbf {
+
+enum VelKind { VEL_TC2 = SERBF_FLOW_TC2, VEL_SOLID = SERBF_FLOW_SOLID_BODY,
+               VEL_DEFORM = SERBF_FLOW_DEFORM, VEL_ARRAY = 3 };
+
+template <typename T>
+struct StageArgs {
+    const T *qin;       // gathered stage input     [N][ntr][LANES]
+    const T *qT;        // step-start state          (stages 2-4)
+    T *qout;            // stage output state        (stages 1-3), q_T (stage 4)
+    T *F;               // accumulator / RHS output
+    const int *idx;     // [N][n]
+    const T *Wx, *Wy, *WL;
+    const T *basis;     // [N][kBasis]
+    const T *lev;       // [4][LANES]
+    const int *vstart;  // [LANES]
+    const T *vW;        // [LANES][kMaxNv]
+    const T *tscal;     // time table
+    const int *base;    // step counter
+    const T *uarr;      // VEL_ARRAY: Up [N][3][LANES]
+    const T *gamma;     // per-point hyperviscosity coefficient (hv_mode 1)
+    int N, n, Nv, nv;
+    int flow, ref_mode, hv_mode;
+    int step_off;
+    int nblk;           // grid size (for the XCD remap)
+    T dt, hdt, dt6, hv_coef, alpha;
+};
+
+__device__ __forceinline__ int xcd_remap(int b, int nblk)
+{
+    // Hardware dispatches work-group b to XCD b % 8.  Give XCD x the contiguous
+    // block range [x*q + min(x,r), ...) so each XCD's L2 serves one region.
+    const int q = nblk >> 3, r = nblk & 7;
+    const int x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+template <typename T, int LANES, int STAGE, int NTR, bool HV>
+__global__ void __launch_bounds__(256) stage_kernel(StageArgs<T> a)
+{
+    constexpr int NPW = 64 / LANES;
+    const int lane = threadIdx.x & 63;
+    const int blk = xcd_remap(blockIdx.x, a.nblk);
+    const int wave = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int sub = lane / LANES;
+    const int v = lane % LANES;
+    int p = wave * NPW + sub;
+    if (NPW == 1) {
+        p = __builtin_amdgcn_readfirstlane(p);
+        if (p >= a.N)
+            return;
+    }
+    const bool node_ok = p < a.N;
+    const int pc = node_ok ? p : a.N - 1;
+    const bool lvl_ok = v < a.Nv;
+    const int n = a.n;
+
+    const int *__restrict__ ip = a.idx + (size_t)pc * n;
+    const T *__restrict__ wxp = a.Wx + (size_t)pc * n;
+    const T *__restrict__ wyp = a.Wy + (size_t)pc * n;
+    const T *__restrict__ wlp = a.WL + (size_t)pc * n;
+
+    T ax[NTR], ay[NTR], al[NTR], self[NTR];
+#pragma unroll
+    for (int t = 0; t < NTR; t++) {
+        ax[t] = T(0); ay[t] = T(0); al[t] = T(0); self[t] = T(0);
+    }
+
+    // neighbour 0 is the node itself (self-first stencils)
+    {
+        const int nb = ip[0];
+        const T wx = wxp[0], wy = wyp[0];
+        const T *col = a.qin + (size_t)nb * NTR * LANES + v;
+#pragma unroll
+        for (int t = 0; t < NTR; t++) {
+            const T b = col[t * LANES];
+            self[t] = b;
+            ax[t] = wx * b;
+            ay[t] = wy * b;
+            if (HV) al[t] = wlp[0] * b;
+        }
+    }
+#pragma unroll 4
+    for (int s = 1; s < n; s++) {
+        const int nb = ip[s];
+        const T wx = wxp[s], wy = wyp[s];
+        T wl = T(0);
+        if (HV) wl = wlp[s];
+        const T *col = a.qin + (size_t)nb * NTR * LANES + v;
+#pragma unroll
+        for (int t = 0; t < NTR; t++) {
+            const T b = col[t * LANES];
+            ax[t] = fma(wx, b, ax[t]);
+            ay[t] = fma(wy, b, ay[t]);
+            if (HV) al[t] = fma(wl, b, al[t]);
+        }
+    }
+
+    // vertical stencil on the own column via cross-lane permutes
+    T az[NTR];
+#pragma unroll
+    for (int t = 0; t < NTR; t++) az[t] = T(0);
+    {
+        const int vs = lvl_ok ? a.vstart[v] : 0;
+        const T *wv = a.vW + v * kMaxNv;
+        for (int j = 0; j < a.nv; j++) {
+            const int src = vs + j;
+            const bool ok = lvl_ok && src >= 0 && src < a.Nv;
+            const int sl = sub * LANES + (ok ? src : 0);
+            const T w = ok ? wv[j] : T(0);
+#pragma unroll
+            for (int t = 0; t < NTR; t++) {
+                const T b = __shfl(self[t], sl, 64);
+                az[t] = fma(w, b, az[t]);
+            }
+        }
+    }
+
+    // velocity in the node's rotated frame at this level
+    const T *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
+    const T *B = a.basis + (size_t)pc * kBasis;
+    const T r_inv = a.lev[v];
+    T upx, upy, upz;
+    if (a.flow == VEL_TC2) {
+        const T wt = ts[0];
+        const T ga = wt * a.lev[LANES + v], gb = wt * a.lev[2 * LANES + v];
+        upx = B[0] + (ga * B[3] + gb * B[6]);
+        upy = B[1] + (ga * B[4] + gb * B[7]);
+        upz = a.ref_mode ? B[2] + (ga * B[5] + gb * B[8]) : gb * B[9];
+    } else if (a.flow == VEL_DEFORM) {
+        const T c = ts[0], C2 = ts[1], S2 = ts[2];
+        const T cl = T(0.5) - T(0.5) * (B[9] * C2 + B[10] * S2);
+        const T cp = B[10] * C2 - B[9] * S2;
+        const T w = c * a.lev[LANES + v];
+        upx = B[0] + c * (cl * B[3] + cp * B[6]) + w * B[11];
+        upy = B[1] + c * (cl * B[4] + cp * B[7]) + w * B[12];
+        upz = a.ref_mode ? B[2] + c * (cl * B[5] + cp * B[8]) + w * B[13] : w;
+    } else if (a.flow == VEL_SOLID) {
+        upx = B[0];
+        upy = B[1];
+        upz = a.ref_mode ? B[2] : T(0);
+    } else {
+        const T *u = a.uarr + (size_t)pc * 3 * LANES + v;
+        upx = u[0];
+        upy = u[LANES];
+        upz = u[2 * LANES];
+    }
+
+    if (!(node_ok && lvl_ok))
+        return;
+    const size_t base = (size_t)p * NTR * LANES + v;
+#pragma unroll
+    for (int t = 0; t < NTR; t++) {
+        const T gx = ax[t] * r_inv, gy = ay[t] * r_inv;
+        T f = a.alpha * ((upx * gx + upy * gy) + upz * az[t]);
+        if (HV) f += (a.hv_mode ? a.gamma[base + t * LANES] : a.hv_coef) * al[t];
+        const size_t o = base + t * LANES;
+        if (STAGE == 0) {
+            a.F[o] = f;
+        } else if (STAGE == 1) {
+            a.F[o] = f;
+            a.qout[o] = self[t] + a.hdt * f;
+        } else if (STAGE == 2) {
+            a.F[o] = a.F[o] + T(2) * f;
+            a.qout[o] = a.qT[o] + a.hdt * f;
+        } else if (STAGE == 3) {
+            a.F[o] = a.F[o] + T(2) * f;
+            a.qout[o] = a.qT[o] + a.dt * f;
+        } else {
+            a.qout[o] = a.qT[o] + a.dt6 * (a.F[o] + T(1) * f);
+        }
+    }
+}
+
+__global__ void bump_kernel(int *base, int by)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *base += by;
+}
+
+// ---------------------------------------------------------------------------
+// Per-node velocity basis (FP64, once per configuration).
+// TC2 (test_cases.c:106-148, 186-228): U = f3 e_lam + w_t (g2(h) f2 e_phi + g1(h) f1 e_r)
+//   basis: [0..2] f3 H e_lam, [3..5] f2 H e_phi, [6..8] f1 H e_r, [9] f1
+//   f3 = u0 cos(phi), f2 = cos(phi) sin(K phi), f1 = -2 sin(K phi) sin(phi) + K cos(K phi) cos(phi)
+// SOLID: [0..2] H (u0 Omega x x), Omega = (-sin a, 0, cos a)
+// DEFORM: U = (u0 cos phi + c_t k sin^2(lam') sin 2phi) e_lam + c_t k sin(2 lam') cos phi e_phi
+//           + c_t w(h) e_r, lam' = lam - theta_t; with cos 2lam' and sin 2lam' expanded:
+//   [0..2] u0 cos(phi) H e_lam, [3..5] k sin(2phi) H e_lam, [6..8] k cos(phi) H e_phi,
+//   [9] cos 2lam, [10] sin 2lam, [11..13] H e_r
+__global__ void basis_kernel(int N, const double *x, const double *y, const double *z,
+                             const double *lam_in, const double *phi_in, serbf_config cfg,
+                             double *basis)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N)
+        return;
+    const double X = x[p], Y = y[p], Z = z[p];
+    double lam, phi;
+    if (lam_in) {
+        lam = lam_in[p];
+        phi = phi_in[p];
+    } else {
+        lam = atan2(Y, X);
+        if (lam < 0.0) lam += 2.0 * M_PI;
+        phi = atan2(Z, sqrt(X * X + Y * Y));
+    }
+    const bool ref = cfg.mode == SERBF_MODE_REFERENCE;
+    double H[9];
+    householder_dev(X, Y, Z, !ref, H);
+    const double cl = cos(lam), sl = sin(lam), cph = cos(phi), sph = sin(phi);
+    const double el[3] = {-sl, cl, 0.0};
+    const double ep[3] = {-sph * cl, -sph * sl, cph};
+    const double er[3] = {cph * cl, cph * sl, sph};
+    double Hel[3], Hep[3], Her[3];
+    for (int r = 0; r < 3; r++) {
+        Hel[r] = H[3 * r] * el[0] + H[3 * r + 1] * el[1] + H[3 * r + 2] * el[2];
+        Hep[r] = H[3 * r] * ep[0] + H[3 * r + 1] * ep[1] + H[3 * r + 2] * ep[2];
+        Her[r] = H[3 * r] * er[0] + H[3 * r + 1] * er[1] + H[3 * r + 2] * er[2];
+    }
+    double b[kBasis];
+    for (int i = 0; i < kBasis; i++) b[i] = 0.0;
+    if (cfg.flow == SERBF_FLOW_TC2) {
+        const double K = cfg.k_def;
+        const double f3 = cfg.u0 * cph, f2 = cph * sin(K * phi);
+        const double f1 = ((-2) * sin(K * phi) * sph) + (K * cos(K * phi) * cph);
+        for (int r = 0; r < 3; r++) {
+            b[r] = f3 * Hel[r];
+            b[3 + r] = f2 * Hep[r];
+            b[6 + r] = f1 * Her[r];
+        }
+        b[9] = f1;
+    } else if (cfg.flow == SERBF_FLOW_SOLID_BODY) {
+        const double ox = -sin(cfg.alpha), oz = cos(cfg.alpha);
+        const double U[3] = {cfg.u0 * (-oz * Y), cfg.u0 * (oz * X - ox * Z), cfg.u0 * (ox * Y)};
+        for (int r = 0; r < 3; r++)
+            b[r] = H[3 * r] * U[0] + H[3 * r + 1] * U[1] + H[3 * r + 2] * U[2];
+    } else {
+        const double k = cfg.k_def;
+        for (int r = 0; r < 3; r++) {
+            b[r] = cfg.u0 * cph * Hel[r];
+            b[3 + r] = k * sin(2.0 * phi) * Hel[r];
+            b[6 + r] = k * cph * Hep[r];
+            b[11 + r] = Her[r];
+        }
+        b[9] = cos(2.0 * lam);
+        b[10] = sin(2.0 * lam);
+    }
+    for (int i = 0; i < kBasis; i++) basis[(size_t)p * kBasis + i] = b[i];
+}
+
+template <typename Tdst>
+__global__ void convert_kernel(size_t n, const double *src, Tdst *dst)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (Tdst)src[i];
+}
+
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+
+double rho_of(double h)
+{
+    const double rho0 = SERBF_P0 / (SERBF_RD * SERBF_T0);
+    return rho0 * std::exp(-(h * SERBF_GRAVITY) / (SERBF_RD * SERBF_T0));
+}
+
+// lev[0][v] = 1/(R+h_v) (layers.c:37-42); lev[1], lev[2] = per-level velocity factors
+void level_tables(const serbf_config &c, int S, std::vector<double> &lev)
+{
+    lev.assign(4 * S, 0.0);
+    const double dh = c.htop / c.Nv;
+    const double rho0 = SERBF_P0 / (SERBF_RD * SERBF_T0);
+    for (int v = 0; v < S; v++) {
+        const double h = dh * (v + .5);
+        lev[v] = 1 / (c.R + h);
+        if (c.flow == SERBF_FLOW_TC2) {
+            const double K = c.k_def, w0 = c.w0;
+            lev[S + v] = -((c.R * w0 * SERBF_PI * rho0) / (K * c.htop * rho_of(h))) *
+                         std::cos(SERBF_PI * h / c.htop);
+            lev[2 * S + v] = (w0 * rho0) / (K * rho_of(h)) * std::sin(SERBF_PI * h / c.htop);
+        } else if (c.flow == SERBF_FLOW_DEFORM) {
+            lev[S + v] = c.w0 * std::sin(SERBF_PI * h / c.htop);
+        }
+    }
+}
+
+// scalars of the flow at time t
+void time_scalars(const serbf_config &c, double t, double out[kTimeScalars])
+{
+    for (int i = 0; i < kTimeScalars; i++) out[i] = 0.0;
+    if (c.flow == SERBF_FLOW_TC2) {
+        out[0] = std::cos(SERBF_PI * (t / c.T));
+    } else if (c.flow == SERBF_FLOW_DEFORM) {
+        const double th = 2.0 * SERBF_PI * t / c.T;
+        out[0] = std::cos(SERBF_PI * t / c.T);
+        out[1] = std::cos(2.0 * th);
+        out[2] = std::sin(2.0 * th);
+    }
+}
+
+template <typename T>
+int upload_converted(serbf_engine *e, T *dst, const double *src_host, size_t n)
+{
+    std::vector<T> tmp(n);
+    for (size_t i = 0; i < n; i++) tmp[i] = (T)src_host[i];
+    SERBF_HIP(hipMemcpyAsync(dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    return SERBF_OK;
+}
+
+template <typename T>
+int alloc_state(serbf_engine *e, StateBuffers<T> &sb)
+{
+    const size_t P = (size_t)e->N * e->ntr * e->S;
+    if (!sb.qT) {
+        for (T **b : {&sb.qT, &sb.qA, &sb.qB, &sb.F}) {
+            SERBF_HIP(hipMalloc((void **)b, P * sizeof(T)));
+            SERBF_HIP(hipMemsetAsync(*b, 0, P * sizeof(T), e->stream));
+        }
+        SERBF_HIP(hipMalloc((void **)&sb.Wx, (size_t)e->N * e->n * sizeof(T)));
+        SERBF_HIP(hipMalloc((void **)&sb.Wy, (size_t)e->N * e->n * sizeof(T)));
+        SERBF_HIP(hipMalloc((void **)&sb.WL, (size_t)e->N * e->n * sizeof(T)));
+        SERBF_HIP(hipMalloc((void **)&sb.basis, (size_t)e->N * kBasis * sizeof(T)));
+        SERBF_HIP(hipMalloc((void **)&sb.lev, 4 * e->S * sizeof(T)));
+        SERBF_HIP(hipMalloc((void **)&sb.vW, e->S * kMaxNv * sizeof(T)));
+    }
+    return SERBF_OK;
+}
+
+template <typename T>
+void free_state(StateBuffers<T> &sb)
+{
+    for (T *b : {sb.qT, sb.qA, sb.qB, sb.F, sb.Wx, sb.Wy, sb.WL, sb.basis, sb.lev, sb.vW, sb.tscal, sb.gamma})
+        if (b) (void)hipFree(b);
+    sb = StateBuffers<T>();
+}
+
+template <typename T>
+int fill_tables(serbf_engine *e, StateBuffers<T> &sb, const double *basis64)
+{
+    const size_t nw = (size_t)e->N * e->n;
+    const int thr = 256, blocks = 2048;
+    if (std::is_same<T, double>::value) {
+        SERBF_HIP(hipMemcpyAsync(sb.Wx, e->d_Wx, nw * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+        SERBF_HIP(hipMemcpyAsync(sb.Wy, e->d_Wy, nw * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+        SERBF_HIP(hipMemcpyAsync(sb.WL, e->d_WL, nw * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+        SERBF_HIP(hipMemcpyAsync(sb.basis, basis64, (size_t)e->N * kBasis * sizeof(double),
+                                 hipMemcpyDeviceToDevice, e->stream));
+    } else {
+        hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream, nw, e->d_Wx, sb.Wx);
+        hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream, nw, e->d_Wy, sb.Wy);
+        hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream, nw, e->d_WL, sb.WL);
+        hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream,
+                           (size_t)e->N * kBasis, basis64, sb.basis);
+        SERBF_HIP(hipGetLastError());
+    }
+    std::vector<double> lev;
+    level_tables(e->cfg, e->S, lev);
+    int rc = upload_converted<T>(e, sb.lev, lev.data(), lev.size());
+    if (rc) return rc;
+    std::vector<double> vw(e->S * kMaxNv, 0.0);
+    for (int v = 0; v < e->cfg.Nv; v++)
+        for (int j = 0; j < e->nv; j++) vw[v * kMaxNv + j] = e->h_vW[v * e->nv + j];
+    return upload_converted<T>(e, sb.vW, vw.data(), vw.size());
+}
+
+template <typename T>
+int ensure_tscal(serbf_engine *e, StateBuffers<T> &sb, int nsteps, double dt, double t0)
+{
+    if (nsteps > e->tcap || !sb.tscal) {
+        const int cap = nsteps > e->tcap ? nsteps : e->tcap;
+        if (e->s64.tscal) { (void)hipFree(e->s64.tscal); e->s64.tscal = nullptr; }
+        if (e->s32.tscal) { (void)hipFree(e->s32.tscal); e->s32.tscal = nullptr; }
+        SERBF_HIP(hipMalloc((void **)&e->s64.tscal, (size_t)cap * 4 * kTimeScalars * sizeof(double)));
+        SERBF_HIP(hipMalloc((void **)&e->s32.tscal, (size_t)cap * 4 * kTimeScalars * sizeof(float)));
+        e->tcap = cap;
+        if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
+    }
+    // stage times t, t+dt/2, t+dt/2, t+dt (wrappers.c:56,67,89)
+    std::vector<double> tab((size_t)nsteps * 4 * kTimeScalars);
+    for (int s = 0; s < nsteps; s++) {
+        const double t = t0 + s * dt;
+        const double tt[4] = {t, t + (dt / 2.0), t + (dt / 2.0), t + dt};
+        for (int k = 0; k < 4; k++) time_scalars(e->cfg, tt[k], &tab[((size_t)s * 4 + k) * kTimeScalars]);
+    }
+    return upload_converted<T>(e, sb.tscal, tab.data(), tab.size());
+}
+
+template <typename T, int LANES, int NTR, bool HV>
+void launch_one(int stage, dim3 grid, const StageArgs<T> &a, hipStream_t s)
+{
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((stage_kernel<T, LANES, 0, NTR, HV>), grid, dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<T, LANES, 1, NTR, HV>), grid, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<T, LANES, 2, NTR, HV>), grid, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((stage_kernel<T, LANES, 3, NTR, HV>), grid, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((stage_kernel<T, LANES, 4, NTR, HV>), grid, dim3(256), 0, s, a); break;
+    }
+}
+
+template <typename T, int LANES>
+void launch_lanes(serbf_engine *e, int stage, bool hv, StageArgs<T> a)
+{
+    const int npw = 64 / LANES;
+    const int nwaves = (e->N + npw - 1) / npw;
+    const int nblk = (nwaves + 3) / 4;
+    a.nblk = nblk;
+    const dim3 grid(nblk);
+    if (e->ntr == 1) {
+        if (hv) launch_one<T, LANES, 1, true>(stage, grid, a, e->stream);
+        else launch_one<T, LANES, 1, false>(stage, grid, a, e->stream);
+    } else {
+        if (hv) launch_one<T, LANES, 4, true>(stage, grid, a, e->stream);
+        else launch_one<T, LANES, 4, false>(stage, grid, a, e->stream);
+    }
+}
+
+template <typename T>
+int launch_stage(serbf_engine *e, int stage, bool hv, const StageArgs<T> &a)
+{
+    switch (e->S) {
+    case 16: launch_lanes<T, 16>(e, stage, hv, a); break;
+    case 32: launch_lanes<T, 32>(e, stage, hv, a); break;
+    default: launch_lanes<T, 64>(e, stage, hv, a); break;
+    }
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
+template <typename T>
+StageArgs<T> base_args(serbf_engine *e, StateBuffers<T> &sb, double dt)
+{
+    StageArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.idx = e->d_idx;
+    a.Wx = sb.Wx; a.Wy = sb.Wy; a.WL = sb.WL;
+    a.basis = sb.basis;
+    a.lev = sb.lev;
+    a.vstart = e->d_vstart;
+    a.vW = sb.vW;
+    a.tscal = sb.tscal;
+    a.base = e->d_base;
+    a.gamma = sb.gamma;
+    a.N = e->N; a.n = e->n; a.Nv = e->cfg.Nv; a.nv = e->nv;
+    a.flow = e->cfg.flow;
+    a.ref_mode = e->cfg.mode == SERBF_MODE_REFERENCE;
+    a.hv_mode = e->hv_mode;
+    a.dt = (T)dt; a.hdt = (T)(dt / 2.0); a.dt6 = (T)(dt / 6.0);
+    a.hv_coef = (T)e->cfg.hv_coef;
+    a.alpha = (T)-1.0;
+    return a;
+}
+
+bool hv_enabled(serbf_engine *e)
+{
+    return e->cfg.hv_order > 0 && (e->cfg.hv_coef != 0.0 || e->hv_mode == 1);
+}
+
+template <typename T>
+int enqueue_step(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off)
+{
+    const bool hv = hv_enabled(e);
+    StageArgs<T> a = base_args<T>(e, sb, dt);
+    a.step_off = step_off;
+    a.qT = sb.qT;
+    a.F = sb.F;
+    int rc;
+    a.qin = sb.qT; a.qout = sb.qB;
+    if ((rc = launch_stage<T>(e, 1, hv, a))) return rc;
+    a.qin = sb.qB; a.qout = sb.qA;
+    if ((rc = launch_stage<T>(e, 2, hv, a))) return rc;
+    a.qin = sb.qA; a.qout = sb.qB;
+    if ((rc = launch_stage<T>(e, 3, hv, a))) return rc;
+    a.qin = sb.qB; a.qout = sb.qT;
+    return launch_stage<T>(e, 4, hv, a);
+}
+
+constexpr int kGraphSteps = 8;
+
+template <typename T>
+int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, double dt, double t0, bool timed,
+          double *ms, double *stage_ms)
+{
+    int rc = ensure_tscal<T>(e, sb, nsteps, dt, t0);
+    if (rc) return rc;
+    const int zero = 0;
+    SERBF_HIP(hipMemcpyAsync(e->d_base, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    // the graph of kGraphSteps steps reads the step counter and bumps it at the end
+    const void *key = (const void *)sb.tscal;
+    const int hvk = (hv_enabled(e) ? 1 : 0) + 2 * e->hv_mode;
+    if (nsteps >= kGraphSteps && (!e->graph.exec || e->graph.precision != precision || e->graph.key != key ||
+                                  e->graph.dt != dt || e->graph.hv != hvk)) {
+        if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
+        hipGraph_t g;
+        SERBF_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        for (int j = 0; j < kGraphSteps && !rc; j++) rc = enqueue_step<T>(e, sb, dt, j);
+        hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(64), 0, e->stream, e->d_base, kGraphSteps);
+        hipError_t ce = hipStreamEndCapture(e->stream, &g);
+        if (rc) return rc;
+        SERBF_HIP(ce);
+        SERBF_HIP(hipGraphInstantiate(&e->graph.exec, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        e->graph.precision = precision;
+        e->graph.key = key;
+        e->graph.G = kGraphSteps;
+        e->graph.dt = dt;
+        e->graph.hv = hvk;
+    }
+    if (timed) SERBF_HIP(hipEventRecord(e->ev0, e->stream));
+    const int full = nsteps / kGraphSteps;
+    for (int r = 0; r < full; r++) SERBF_HIP(hipGraphLaunch(e->graph.exec, e->stream));
+    const int rem = nsteps - full * kGraphSteps;
+    for (int j = 0; j < rem; j++)
+        if ((rc = enqueue_step<T>(e, sb, dt, j))) return rc;
+    if (timed) SERBF_HIP(hipEventRecord(e->ev1, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    if (timed) {
+        float t = 0.f;
+        SERBF_HIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
+        *ms = t;
+        *stage_ms = nsteps > 0 ? t / (4.0 * nsteps) : 0.0;
+    }
+    return SERBF_OK;
+}
+
+}  // namespace
+
+// Allocate the FP64 time-loop buffers and fill the per-node/per-level tables.
+int rk4_prepare(serbf_engine *e)
+{
+    SERBF_CHECK(e->configured && e->have_stencils && e->have_weights, SERBF_ERR_STATE,
+                "rk4: configure, stencils and weights are required first");
+    int rc = alloc_state<double>(e, e->s64);
+    if (rc) return rc;
+    double *basis;
+    SERBF_HIP(hipMalloc((void **)&basis, (size_t)e->N * kBasis * sizeof(double)));
+    hipLaunchKernelGGL(basis_kernel, dim3((e->N + 255) / 256), dim3(256), 0, e->stream, e->N, e->d_x, e->d_y,
+                       e->d_z, e->d_lam, e->d_phi, e->cfg, basis);
+    SERBF_HIP(hipGetLastError());
+    rc = fill_tables<double>(e, e->s64, basis);
+    if (!rc) {
+        rc = alloc_state<float>(e, e->s32);
+        if (!rc) rc = fill_tables<float>(e, e->s32, basis);
+    }
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    (void)hipFree(basis);
+    if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
+    e->f32_ready = rc == SERBF_OK;
+    return rc;
+}
+
+int convert_to_f32(serbf_engine *e, size_t n, const double *src, float *dst)
+{
+    hipLaunchKernelGGL(convert_kernel<float>, dim3(2048), dim3(256), 0, e->stream, n, src, dst);
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
+void rk4_free(serbf_engine *e)
+{
+    free_state<double>(e->s64);
+    free_state<float>(e->s32);
+    if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
+    e->tcap = 0;
+}
+
+int rk4_rhs(serbf_engine *e, double t, double *F_host)
+{
+    SERBF_CHECK(e->have_state, SERBF_ERR_STATE, "rhs: no state");
+    double ts[4 * kTimeScalars];
+    for (int k = 0; k < 4; k++) time_scalars(e->cfg, t, ts + k * kTimeScalars);
+    double *d_ts;
+    SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
+    SERBF_HIP(hipMemcpyAsync(d_ts, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
+    StageArgs<double> a = base_args<double>(e, e->s64, 0.0);
+    a.tscal = d_ts;
+    a.base = e->d_base + 1;   // always zero
+    a.qin = e->s64.qT;
+    a.F = e->s64.qA;
+    int rc = launch_stage<double>(e, 0, hv_enabled(e), a);
+    if (!rc) {
+        const size_t P = (size_t)e->N * e->ntr;
+        SERBF_HIP(hipMemcpy2DAsync(F_host, e->cfg.Nv * sizeof(double), e->s64.qA, e->S * sizeof(double),
+                                   e->cfg.Nv * sizeof(double), P, hipMemcpyDeviceToHost, e->stream));
+        SERBF_HIP(hipStreamSynchronize(e->stream));
+    }
+    (void)hipFree(d_ts);
+    return rc;
+}
+
+int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha, const double *q_host,
+                            double *F_host)
+{
+    SERBF_CHECK(e->s64.qT, SERBF_ERR_STATE, "eval: buffers not prepared");
+    SERBF_CHECK(e->ntr == 1, SERBF_ERR_UNSUPPORTED, "eval: one tracer");
+    const int Nv = e->cfg.Nv, S = e->S, N = e->N;
+    double *d_u;
+    SERBF_HIP(hipMalloc((void **)&d_u, (size_t)N * 3 * S * sizeof(double)));
+    SERBF_HIP(hipMemsetAsync(d_u, 0, (size_t)N * 3 * S * sizeof(double), e->stream));
+    SERBF_HIP(hipMemcpy2DAsync(d_u, S * sizeof(double), Up_host, Nv * sizeof(double), Nv * sizeof(double),
+                               (size_t)N * 3, hipMemcpyHostToDevice, e->stream));
+    SERBF_HIP(hipMemcpy2DAsync(e->s64.qB, S * sizeof(double), q_host, Nv * sizeof(double), Nv * sizeof(double),
+                               (size_t)N, hipMemcpyHostToDevice, e->stream));
+    double ts[4 * kTimeScalars] = {0};
+    double *d_ts;
+    SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
+    SERBF_HIP(hipMemcpyAsync(d_ts, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
+    StageArgs<double> a = base_args<double>(e, e->s64, 0.0);
+    a.tscal = d_ts;
+    a.base = e->d_base + 1;
+    a.flow = VEL_ARRAY;
+    a.uarr = d_u;
+    a.alpha = alpha;
+    a.qin = e->s64.qB;
+    a.F = e->s64.qA;
+    int rc = launch_stage<double>(e, 0, false, a);
+    if (!rc) {
+        SERBF_HIP(hipMemcpy2DAsync(F_host, Nv * sizeof(double), e->s64.qA, S * sizeof(double), Nv * sizeof(double),
+                                   (size_t)N, hipMemcpyDeviceToHost, e->stream));
+        SERBF_HIP(hipStreamSynchronize(e->stream));
+    }
+    (void)hipFree(d_ts);
+    (void)hipFree(d_u);
+    return rc;
+}
+
+int rk4_run(serbf_engine *e, int precision, int nsteps, double dt, double t0, bool timed, double *ms,
+            double *stage_ms)
+{
+    SERBF_CHECK(e->have_state, SERBF_ERR_STATE, "run: no state");
+    SERBF_CHECK(nsteps >= 0, SERBF_ERR_ARG, "run: nsteps < 0");
+    if (precision == SERBF_F32) {
+        SERBF_CHECK(e->f32_ready, SERBF_ERR_STATE, "run: FP32 tables not ready");
+        return run_t<float>(e, e->s32, precision, nsteps, dt, t0, timed, ms, stage_ms);
+    }
+    return run_t<double>(e, e->s64, precision, nsteps, dt, t0, timed, ms, stage_ms);
+}
+
+}  // namespace serbf

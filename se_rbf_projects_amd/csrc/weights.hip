@@ -1,0 +1,470 @@
+// Batched per-node RBF-FD weight solves (gfx950), the body of get_rbffd_DMs
+// (rbffd_DMs.c:98-282 of the reference).
+//
+// One persistent launch factors every node's saddle system.  A wavefront owns one
+// system at a time: it gathers the stencil, rotates it with the node's
+// Householder reflector, assembles [Phi P; P^T 0] and the right-hand sides in LDS
+// (lane = row, rows lane + 64 r), factors it with partial pivoting (first maximal
+// |a_ik|, full-row swaps as LAPACK dgesv), solves, and applies one step of
+// iterative refinement whose residual b - A x re-evaluates A from the
+// coordinates (the matrix itself is never stored in HBM).  Work is pulled from a
+// global counter so uneven system sizes balance across the chip.
+//
+// Systems per node:
+//   reference mode: PHS r^k (k = phs_order) + degree-5 tangent monomials, RHS
+//     d/dx', d/dy' of r^k at the centre (rbffd_DMs.c:175-215, 240); Gaussian +
+//     constant with RHS Delta^k by the Laguerre recursion (rbffd_DMs.c:188,198,
+//     223-228, 243, 314-328); coordinates unscaled.
+//   config mode: the stencil is shifted to the centre and scaled by its radius s;
+//     PHS r^k + degree-d monomials for the gradient (weights / s) and PHS r^K +
+//     the same monomials with RHS Delta^k r^K = prod (K-2j)^2 r^(K-2k) for the
+//     hyperviscosity operator (weights / s^2k).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_util.h"
+#include "engine.h"
+
+namespace serbf {
+
+struct WArgs {
+    int N, n, ref_mode;
+    int phs, deg, hv, hvphs;
+    double eps;
+    const double *x, *y, *z;
+    const int *idx;
+    double *Wx, *Wy, *WL;
+    int *status;
+    int *counter;
+};
+
+__device__ __forceinline__ double phs_pow(double r, int k) { return pow(r, (double)k); }
+
+// 2D Delta^k of the Gaussian exp(-(eps r)^2) by the Laguerre recursion (rbffd_DMs.c:314-328)
+__device__ double lap_gauss(double r, double eps, int k)
+{
+    double epr2 = pow(eps * r, 2.0);
+    double pm = 1.0, p = 4 * epr2 - 4;
+    if (k == 0) p = 1.0;
+    for (int i = 1; i < k; i++) {
+        double pn = 4 * (epr2 - (2 * i + 1)) * p - (16 * pow((double)i, 2.0)) * pm;
+        pm = p;
+        p = pn;
+    }
+    return pow(eps, 2.0 * k) * p * exp(-pow(r * eps, 2.0));
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS layout of one wavefront's system (doubles):
+//   A   [MAXM][LD]  augmented matrix [A | B] (LD >= MAXM + 3, odd)
+//   X   [3][MAXN]   stencil coordinates in the solve frame
+//   B0  [MAXM][3]   right-hand sides (kept for the residual)
+//   XS  [MAXM][3]   solution
+//   piv [MAXM]      pivot rows (int)
+template <int MAXM>
+struct Lds {
+    static constexpr int LD = ((MAXM + 3) | 1);
+    static constexpr int MAXN = MAXM;
+    static constexpr int doubles = MAXM * LD + 3 * MAXN + 2 * MAXM * 3;
+    static constexpr int bytes = doubles * 8 + MAXM * 4;
+};
+
+// Matrix entry A_ij of the current system (rows/cols < m); kind 0: gradient PHS system,
+// kind 1: Gaussian (reference) or PHS r^K (config) Delta^k system.
+__device__ __forceinline__ double entry(int i, int j, int n, int m, int kind, const WArgs &a, const double *X,
+                                        int MAXN)
+{
+    const bool gauss = kind == 1 && a.ref_mode;
+    if (i < n && j < n) {
+        const double dx = X[j] - X[i], dy = X[MAXN + j] - X[MAXN + i], dz = X[2 * MAXN + j] - X[2 * MAXN + i];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (gauss) return exp(-pow(r * a.eps, 2.0));
+        return phs_pow(r, kind == 0 ? a.phs : a.hvphs);
+    }
+    if (i >= n && j >= n) return 0.0;
+    const int node = i < n ? i : j;
+    const int t = i < n ? j - n : i - n;
+    if (gauss) return 1.0;
+    // monomial t in the order x^(p-q) y^q, p = 0..deg, q = 0..p
+    int p = 0;
+    while ((p + 1) * (p + 2) / 2 <= t) p++;
+    const int q = t - p * (p + 1) / 2;
+    return pow(X[node], (double)(p - q)) * pow(X[MAXN + node], (double)q);
+}
+
+template <int MAXM>
+__device__ int factor_solve(double *A, double *X, double *B0, double *XS, int *piv, int n, int m, int nrhs,
+                            int kind, const WArgs &a, int lane)
+{
+    constexpr int LD = Lds<MAXM>::LD, MAXN = Lds<MAXM>::MAXN, R = (MAXM + 63) / 64;
+    int info = 0;
+    // ---- LU with partial pivoting on [A | B] ----
+    for (int k = 0; k < m; k++) {
+        double best = -1.0;
+        int bi = m;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = lane + 64 * r;
+            if (i >= k && i < m) {
+                const double v = fabs(A[i * LD + k]);
+                if (v > best) { best = v; bi = i; }
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        const int p = __builtin_amdgcn_readfirstlane(bi);
+        if (lane == 0) piv[k] = p;
+        if (!(best > 0.0)) {
+            if (!info) info = k + 1;
+            wave_sync();
+            continue;
+        }
+        if (p != k)
+            for (int j = lane; j < m + nrhs; j += 64) {
+                const double t = A[k * LD + j];
+                A[k * LD + j] = A[p * LD + j];
+                A[p * LD + j] = t;
+            }
+        wave_sync();
+        const double pv = A[k * LD + k];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = lane + 64 * r;
+            if (i > k && i < m) {
+                const double l = A[i * LD + k] / pv;
+                A[i * LD + k] = l;
+                if (l != 0.0)
+                    for (int j = k + 1; j < m + nrhs; j++) A[i * LD + j] -= l * A[k * LD + j];
+            }
+        }
+        wave_sync();
+    }
+    if (info) return info;
+    // ---- back substitution U x = y (y in the augmented columns) ----
+    for (int i = m - 1; i >= 0; i--) {
+        if (lane < nrhs) A[i * LD + m + lane] /= A[i * LD + i];
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int row = lane + 64 * r;
+            if (row < i) {
+                const double u = A[row * LD + i];
+                for (int c = 0; c < nrhs; c++) A[row * LD + m + c] -= u * A[i * LD + m + c];
+            }
+        }
+        wave_sync();
+    }
+    for (int i = lane; i < m; i += 64)
+        for (int c = 0; c < nrhs; c++) XS[i * 3 + c] = A[i * LD + m + c];
+    wave_sync();
+    // ---- one refinement step: r = b - A x with A re-evaluated, solve with the factors ----
+    double res[R][3];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int i = lane + 64 * r;
+        for (int c = 0; c < 3; c++) res[r][c] = 0.0;
+        if (i < m) {
+            double acc[3] = {B0[i * 3], B0[i * 3 + 1], B0[i * 3 + 2]};
+            for (int j = 0; j < m; j++) {
+                const double aij = entry(i, j, n, m, kind, a, X, MAXN);
+                for (int c = 0; c < nrhs; c++) acc[c] -= aij * XS[j * 3 + c];
+            }
+            for (int c = 0; c < nrhs; c++) res[r][c] = acc[c];
+        }
+    }
+    // store the residual in the augmented columns, apply the row swaps, then L and U solves
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int i = lane + 64 * r;
+        if (i < m)
+            for (int c = 0; c < nrhs; c++) A[i * LD + m + c] = res[r][c];
+    }
+    wave_sync();
+    if (lane < nrhs)
+        for (int k = 0; k < m; k++) {
+            const int p = piv[k];
+            if (p != k) {
+                const double t = A[k * LD + m + lane];
+                A[k * LD + m + lane] = A[p * LD + m + lane];
+                A[p * LD + m + lane] = t;
+            }
+        }
+    wave_sync();
+    for (int k = 0; k < m; k++) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int row = lane + 64 * r;
+            if (row > k && row < m) {
+                const double l = A[row * LD + k];
+                for (int c = 0; c < nrhs; c++) A[row * LD + m + c] -= l * A[k * LD + m + c];
+            }
+        }
+        wave_sync();
+    }
+    for (int i = m - 1; i >= 0; i--) {
+        if (lane < nrhs) A[i * LD + m + lane] /= A[i * LD + i];
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int row = lane + 64 * r;
+            if (row < i) {
+                const double u = A[row * LD + i];
+                for (int c = 0; c < nrhs; c++) A[row * LD + m + c] -= u * A[i * LD + m + c];
+            }
+        }
+        wave_sync();
+    }
+    for (int i = lane; i < m; i += 64)
+        for (int c = 0; c < nrhs; c++) XS[i * 3 + c] += A[i * LD + m + c];
+    wave_sync();
+    return 0;
+}
+
+template <int MAXM>
+__device__ void assemble(double *A, double *X, double *B0, int n, int m, int nrhs, int kind, const WArgs &a,
+                         int lane)
+{
+    constexpr int LD = Lds<MAXM>::LD, MAXN = Lds<MAXM>::MAXN;
+    for (int i = lane; i < m; i += 64) {
+        for (int j = 0; j < m; j++) A[i * LD + j] = entry(i, j, n, m, kind, a, X, MAXN);
+        double b[3] = {0.0, 0.0, 0.0};
+        if (i < n) {
+            // derivative at the centre (stencil node 0) of the basis function centred at node i
+            const double dx = X[0] - X[i], dy = X[MAXN] - X[MAXN + i], dz = X[2 * MAXN] - X[2 * MAXN + i];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (kind == 0) {
+                const int k = a.phs;
+                b[0] = k * dx * pow(r, (double)(k - 2));
+                b[1] = k * dy * pow(r, (double)(k - 2));
+            } else if (a.ref_mode) {
+                b[0] = lap_gauss(r, a.eps, a.hv);
+            } else {
+                double c = 1.0;
+                for (int jj = 0; jj < a.hv; jj++) c *= (double)(a.hvphs - 2 * jj) * (double)(a.hvphs - 2 * jj);
+                b[0] = c * pow(r, (double)(a.hvphs - 2 * a.hv));
+            }
+        } else if (kind == 0) {
+            // d/dx' of x' and d/dy' of y' at the origin of the tangent plane
+            b[0] = (i - n == 1) ? 1.0 : 0.0;
+            b[1] = (i - n == 2) ? 1.0 : 0.0;
+        }
+        for (int c = 0; c < nrhs; c++) {
+            A[i * LD + m + c] = b[c];
+            B0[i * 3 + c] = b[c];
+        }
+        for (int c = nrhs; c < 3; c++) B0[i * 3 + c] = 0.0;
+    }
+    wave_sync();
+}
+
+template <int MAXM>
+__global__ void __launch_bounds__(64) weights_kernel(WArgs a)
+{
+    extern __shared__ double smem[];
+    constexpr int LD = Lds<MAXM>::LD, MAXN = Lds<MAXM>::MAXN;
+    double *A = smem;
+    double *X = A + MAXM * LD;
+    double *B0 = X + 3 * MAXN;
+    double *XS = B0 + MAXM * 3;
+    int *piv = (int *)(XS + MAXM * 3);
+    const int lane = threadIdx.x;
+    const int n = a.n;
+    const int M = a.ref_mode ? 21 : (a.deg + 1) * (a.deg + 2) / 2;
+    const int m1 = n + M;
+    const int m2 = a.ref_mode ? n + 1 : n + M;
+    for (;;) {
+        int node = 0;
+        if (lane == 0) node = atomicAdd(a.counter, 1);
+        node = __builtin_amdgcn_readfirstlane(__shfl(node, 0, 64));
+        if (node >= a.N) break;
+        double H[9];
+        householder_dev(a.x[node], a.y[node], a.z[node], !a.ref_mode, H);
+        // gather and rotate the stencil (rbffd_DMs.c:159-168)
+        const int *ip = a.idx + (size_t)node * n;
+        for (int s = lane; s < n; s += 64) {
+            const int q = ip[s];
+            const double px = a.x[q], py = a.y[q], pz = a.z[q];
+            for (int r = 0; r < 3; r++) X[r * MAXN + s] = H[3 * r] * px + H[3 * r + 1] * py + H[3 * r + 2] * pz;
+        }
+        wave_sync();
+        double scale = 1.0;
+        if (!a.ref_mode) {
+            // shift to the centre, scale by the stencil radius
+            const double c0 = X[0], c1 = X[MAXN], c2 = X[2 * MAXN];
+            wave_sync();
+            double rmax = 0.0;
+            for (int s = lane; s < n; s += 64) {
+                const double dx = X[s] - c0, dy = X[MAXN + s] - c1, dz = X[2 * MAXN + s] - c2;
+                X[s] = dx; X[MAXN + s] = dy; X[2 * MAXN + s] = dz;
+                rmax = fmax(rmax, sqrt(dx * dx + dy * dy + dz * dz));
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
+            scale = rmax;
+            wave_sync();
+            for (int s = lane; s < n; s += 64) {
+                X[s] /= scale; X[MAXN + s] /= scale; X[2 * MAXN + s] /= scale;
+            }
+            wave_sync();
+        }
+        int st = 0;
+        // gradient system
+        assemble<MAXM>(A, X, B0, n, m1, 2, 0, a, lane);
+        if (factor_solve<MAXM>(A, X, B0, XS, piv, n, m1, 2, 0, a, lane)) st = 1;
+        for (int s = lane; s < n; s += 64) {
+            a.Wx[(size_t)node * n + s] = st ? 0.0 : XS[s * 3] / scale;
+            a.Wy[(size_t)node * n + s] = st ? 0.0 : XS[s * 3 + 1] / scale;
+        }
+        wave_sync();
+        // Delta^k system
+        if (a.hv > 0) {
+            assemble<MAXM>(A, X, B0, n, m2, 1, 1, a, lane);
+            int bad = factor_solve<MAXM>(A, X, B0, XS, piv, n, m2, 1, 1, a, lane);
+            if (bad && !st) st = 2;
+            const double s2k = pow(scale, 2.0 * a.hv);
+            for (int s = lane; s < n; s += 64) a.WL[(size_t)node * n + s] = bad ? 0.0 : XS[s * 3] / s2k;
+        } else {
+            for (int s = lane; s < n; s += 64) a.WL[(size_t)node * n + s] = 0.0;
+        }
+        if (lane == 0) a.status[node] = st;
+        wave_sync();
+    }
+}
+
+template <int MAXM>
+static int launch_weights(serbf_engine *e, const WArgs &a)
+{
+    const int bytes = Lds<MAXM>::bytes;
+    SERBF_HIP(hipFuncSetAttribute((const void *)weights_kernel<MAXM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  bytes));
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, e->device) == hipSuccess) cus = prop.multiProcessorCount;
+    const int per_cu = (160 * 1024) / bytes > 0 ? (160 * 1024) / bytes : 1;
+    int grid = cus * per_cu;
+    if (grid > e->N) grid = e->N;
+    hipLaunchKernelGGL(weights_kernel<MAXM>, dim3(grid), dim3(64), bytes, e->stream, a);
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
+int weights_run(serbf_engine *e)
+{
+    const serbf_config &c = e->cfg;
+    const int n = e->n;
+    const int M = c.mode == SERBF_MODE_REFERENCE ? 21 : (c.poly_deg + 1) * (c.poly_deg + 2) / 2;
+    const int m = n + M;
+    SERBF_CHECK(m <= 112, SERBF_ERR_UNSUPPORTED, "weights: n + polynomial terms must be <= 112");
+    int rc = alloc_weights(e);
+    if (rc) return rc;
+    rc = ensure_scratch(e, 64);
+    if (rc) return rc;
+    SERBF_HIP(hipMemsetAsync(e->d_scratch, 0, sizeof(int), e->stream));
+    WArgs a;
+    a.N = e->N; a.n = n; a.ref_mode = c.mode == SERBF_MODE_REFERENCE;
+    a.phs = c.phs_order; a.deg = c.poly_deg; a.hv = c.hv_order; a.hvphs = c.hv_phs_order;
+    a.eps = c.hv_eps;
+    a.x = e->d_x; a.y = e->d_y; a.z = e->d_z;
+    a.idx = e->d_idx;
+    a.Wx = e->d_Wx; a.Wy = e->d_Wy; a.WL = e->d_WL;
+    a.status = e->d_status;
+    a.counter = (int *)e->d_scratch;
+    if (m <= 48) rc = launch_weights<48>(e, a);
+    else if (m <= 80) rc = launch_weights<80>(e, a);
+    else rc = launch_weights<112>(e, a);
+    if (rc) return rc;
+    std::vector<int> st(e->N);
+    SERBF_HIP(hipMemcpyAsync(st.data(), e->d_status, e->N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    e->have_weights = true;
+    rk4_free(e);
+    e->have_state = false;
+    for (int s : st)
+        if (s) {
+            set_error("weights: at least one singular system (see status)");
+            return SERBF_ERR_SINGULAR;
+        }
+    return SERBF_OK;
+}
+
+// ---- vertical operator (host, tiny) ----
+static int host_dgesv(int m, double *A, double *b)
+{
+    std::vector<int> piv(m);
+    for (int k = 0; k < m; k++) {
+        int p = k;
+        for (int i = k + 1; i < m; i++)
+            if (std::fabs(A[i * m + k]) > std::fabs(A[p * m + k])) p = i;
+        if (A[p * m + k] == 0.0) return k + 1;
+        if (p != k) {
+            for (int j = 0; j < m; j++) std::swap(A[k * m + j], A[p * m + j]);
+            std::swap(b[k], b[p]);
+        }
+        for (int i = k + 1; i < m; i++) {
+            const double l = A[i * m + k] / A[k * m + k];
+            for (int j = k + 1; j < m; j++) A[i * m + j] -= l * A[k * m + j];
+            b[i] -= l * b[k];
+        }
+    }
+    for (int i = m - 1; i >= 0; i--) {
+        for (int j = i + 1; j < m; j++) b[i] -= A[i * m + j] * b[j];
+        b[i] /= A[i * m + i];
+    }
+    return 0;
+}
+
+// Reference mode: the 7-point 6th-order FD1 / dh on levels v-3..v+3, zero outside the
+// column (config.h:21-22, rbffd_DMs.c:146-154, 412-427).  Config mode: 1D PHS |z|^kv +
+// degree-dv polynomials on the nv levels nearest v (one-sided at the ends), in units of dh.
+int vertical_table(const serbf_config &c, int *start, double *W)
+{
+    const int Nv = c.Nv;
+    const double dh = c.htop / Nv;
+    if (c.mode == SERBF_MODE_REFERENCE) {
+        static const double fd1[7] = {-1.0 / 60.0, 3.0 / 20.0, -3.0 / 4.0, 0.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
+        for (int v = 0; v < Nv; v++) {
+            start[v] = v - 3;
+            for (int j = 0; j < 7; j++) W[v * 7 + j] = fd1[j] / dh;
+        }
+        return 7;
+    }
+    const int nv = c.nv, kv = c.v_phs_order, dv = c.v_poly_deg, m = nv + dv + 1;
+    if (nv < 1 || nv > Nv || nv > kMaxNv || m > 32) return -1;
+    for (int v = 0; v < Nv; v++) {
+        int s0 = v - nv / 2;
+        if (s0 < 0) s0 = 0;
+        if (s0 > Nv - nv) s0 = Nv - nv;
+        start[v] = s0;
+        double zc[kMaxNv];
+        for (int j = 0; j < nv; j++) zc[j] = (double)(s0 + j - v);
+        std::vector<double> A(m * m, 0.0), b(m, 0.0);
+        for (int i = 0; i < nv; i++) {
+            for (int j = 0; j < nv; j++) A[i * m + j] = std::pow(std::fabs(zc[i] - zc[j]), kv);
+            for (int q = 0; q <= dv; q++) {
+                const double p = std::pow(zc[i], q);
+                A[i * m + nv + q] = p;
+                A[(nv + q) * m + i] = p;
+            }
+            b[i] = kv * (0.0 - zc[i]) * std::pow(std::fabs(zc[i]), kv - 2);
+        }
+        for (int q = 0; q <= dv; q++) b[nv + q] = q == 1 ? 1.0 : 0.0;
+        if (host_dgesv(m, A.data(), b.data())) return -1;
+        for (int j = 0; j < nv; j++) W[v * nv + j] = b[j] / dh;
+    }
+    return nv;
+}
+
+}  // namespace serbf

@@ -1,0 +1,228 @@
+"""Python host binding of libserbf.so (the MI355X RBF-FD tracer-advection engine).
+
+A thin ctypes layer over the C-ABI in include/serbf.h: device memory, kernels and
+the time loop all live in the library.  Loading fails loudly when the library has
+not been built; there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .ctypes_api import F32, F64, SerbfConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libserbf.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_SINGULAR, ERR_UNSUPPORTED, ERR_STATE, ERR_NOMEM = range(7)
+IC_TC2, IC_BELL, IC_HILLS, IC_SLOTTED, IC_BELLS2, IC_CONST = range(6)
+
+
+class SerbfError(RuntimeError):
+    def __init__(self, code, where, msg):
+        super().__init__(f"{where}: status {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libserbf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run se_rbf_projects_amd.build.build() first")
+        L = C.CDLL(LIB_PATH)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        sig = {
+            "serbf_last_error": ([], C.c_char_p),
+            "serbf_icos_count": ([i], i),
+            "serbf_icos_nodes": ([i, vp, vp, vp], i),
+            "serbf_jitter_nodes": ([i, vp, vp, vp, d, C.c_ulonglong], i),
+            "serbf_spatial_order": ([i, vp, vp, vp, vp], i),
+            "serbf_lonlat": ([i, vp, vp, vp, vp, vp], None),
+            "serbf_initial_condition": ([i, vp, i, vp, vp, vp, C.c_ulonglong, vp], i),
+            "serbf_create": ([C.POINTER(vp), i], i),
+            "serbf_destroy": ([vp], None),
+            "serbf_set_nodes": ([vp, i, vp, vp, vp], i),
+            "serbf_set_lonlat": ([vp, vp, vp], i),
+            "serbf_knn": ([vp, i], i),
+            "serbf_get_stencils": ([vp, vp, vp], i),
+            "serbf_set_stencils": ([vp, i, vp], i),
+            "serbf_configure": ([vp, vp], i),
+            "serbf_weights": ([vp], i),
+            "serbf_get_weights": ([vp, vp, vp, vp, vp], i),
+            "serbf_set_weights": ([vp, vp, vp, vp], i),
+            "serbf_get_vertical": ([vp, vp, vp, vp], i),
+            "serbf_set_state": ([vp, vp], i),
+            "serbf_get_state": ([vp, i, vp], i),
+            "serbf_rhs": ([vp, d, vp], i),
+            "serbf_run": ([vp, i, i, d, d], i),
+            "serbf_run_timed": ([vp, i, i, d, d, vp, vp], i),
+            "serbf_sync_f32_from_f64": ([vp], i),
+            "serbf_norms": ([vp, i, vp, vp], i),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _check(rc, where):
+    if rc != OK:
+        raise SerbfError(rc, where, lib().serbf_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def icos_nodes(level):
+    """Icosahedral node set of the given subdivision level, float64 [N, 3]."""
+    N = lib().serbf_icos_count(level)
+    if N < 0:
+        raise ValueError("level out of range")
+    x, y, z = (np.empty(N) for _ in range(3))
+    _check(lib().serbf_icos_nodes(level, _p(x), _p(y), _p(z)), "icos_nodes")
+    return np.stack([x, y, z], axis=1)
+
+
+def jitter_nodes(xyz, frac, seed):
+    x, y, z = (np.ascontiguousarray(xyz[:, k], dtype=np.float64).copy() for k in range(3))
+    _check(lib().serbf_jitter_nodes(len(x), _p(x), _p(y), _p(z), float(frac), int(seed)), "jitter_nodes")
+    return np.stack([x, y, z], axis=1)
+
+
+def spatial_order(xyz):
+    x, y, z = (np.ascontiguousarray(xyz[:, k], dtype=np.float64) for k in range(3))
+    perm = np.empty(len(x), np.int32)
+    _check(lib().serbf_spatial_order(len(x), _p(x), _p(y), _p(z), _p(perm)), "spatial_order")
+    return perm
+
+
+def initial_condition(kind, cfg, xyz, seed=0):
+    x, y, z = (np.ascontiguousarray(xyz[:, k], dtype=np.float64) for k in range(3))
+    q = np.empty((len(x), cfg.Nv))
+    _check(lib().serbf_initial_condition(kind, C.byref(cfg), len(x), _p(x), _p(y), _p(z), int(seed), _p(q)),
+           "initial_condition")
+    return q
+
+
+class Engine:
+    """Device-resident RBF-FD advection engine on one GPU."""
+
+    def __init__(self, device=0):
+        self._e = C.c_void_p()
+        _check(lib().serbf_create(C.byref(self._e), int(device)), "create")
+        self.N = 0
+        self.n = 0
+        self.cfg = None
+
+    def close(self):
+        if self._e:
+            lib().serbf_destroy(self._e)
+            self._e = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_nodes(self, xyz, lonlat=None):
+        xyz = _f64(xyz)
+        self.N = len(xyz)
+        self._xyz = [np.ascontiguousarray(xyz[:, k]) for k in range(3)]
+        _check(lib().serbf_set_nodes(self._e, self.N, *[_p(a) for a in self._xyz]), "set_nodes")
+        if lonlat is not None:
+            lam, phi = (_f64(a) for a in lonlat)
+            _check(lib().serbf_set_lonlat(self._e, _p(lam), _p(phi)), "set_lonlat")
+
+    def knn(self, n):
+        _check(lib().serbf_knn(self._e, int(n)), "knn")
+        self.n = n
+
+    def stencils(self):
+        idx = np.empty((self.N, self.n), np.int32)
+        D = np.empty((self.N, self.n))
+        _check(lib().serbf_get_stencils(self._e, _p(idx), _p(D)), "get_stencils")
+        return idx, D
+
+    def set_stencils(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        self.n = idx.shape[1]
+        _check(lib().serbf_set_stencils(self._e, self.n, _p(idx)), "set_stencils")
+
+    def configure(self, cfg):
+        assert isinstance(cfg, SerbfConfig)
+        self.cfg = cfg
+        _check(lib().serbf_configure(self._e, C.byref(cfg)), "configure")
+
+    @property
+    def ntr(self):
+        return max(1, self.cfg.ntracers)
+
+    def weights(self, allow_singular=False):
+        rc = lib().serbf_weights(self._e)
+        if not (allow_singular and rc == ERR_SINGULAR):
+            _check(rc, "weights")
+        return rc
+
+    def get_weights(self):
+        Dx, Dy, L = (np.empty((self.N, self.n)) for _ in range(3))
+        st = np.empty(self.N, np.int32)
+        _check(lib().serbf_get_weights(self._e, _p(Dx), _p(Dy), _p(L), _p(st)), "get_weights")
+        return Dx, Dy, L, st
+
+    def set_weights(self, Dx, Dy, L=None):
+        Dx, Dy = _f64(Dx), _f64(Dy)
+        L = None if L is None else _f64(L)
+        _check(lib().serbf_set_weights(self._e, _p(Dx), _p(Dy), _p(L)), "set_weights")
+
+    def vertical(self):
+        Nv = self.cfg.Nv
+        start = np.empty(Nv, np.int32)
+        W = np.empty(Nv * 9)
+        nv = C.c_int()
+        _check(lib().serbf_get_vertical(self._e, _p(start), _p(W), C.byref(nv)), "get_vertical")
+        return start, W[: Nv * nv.value].reshape(Nv, nv.value)
+
+    def set_state(self, q):
+        q = _f64(q)
+        assert q.size == self.N * self.ntr * self.cfg.Nv
+        _check(lib().serbf_set_state(self._e, _p(q)), "set_state")
+
+    def state(self, precision=F64):
+        q = np.empty((self.N, self.ntr, self.cfg.Nv)) if self.ntr > 1 else np.empty((self.N, self.cfg.Nv))
+        _check(lib().serbf_get_state(self._e, int(precision), _p(q)), "get_state")
+        return q
+
+    def rhs(self, t):
+        F = np.empty((self.N, self.ntr, self.cfg.Nv)) if self.ntr > 1 else np.empty((self.N, self.cfg.Nv))
+        _check(lib().serbf_rhs(self._e, float(t), _p(F)), "rhs")
+        return F
+
+    def run(self, precision, nsteps, dt, t0=0.0):
+        _check(lib().serbf_run(self._e, int(precision), int(nsteps), float(dt), float(t0)), "run")
+
+    def run_timed(self, precision, nsteps, dt, t0=0.0):
+        ms, st = C.c_double(), C.c_double()
+        _check(lib().serbf_run_timed(self._e, int(precision), int(nsteps), float(dt), float(t0), C.byref(ms),
+                                     C.byref(st)), "run_timed")
+        return ms.value, st.value
+
+    def sync_f32(self):
+        _check(lib().serbf_sync_f32_from_f64(self._e), "sync_f32")
+
+    def norms(self, precision=F64, q_ref=None):
+        out = np.empty(7)
+        qr = None if q_ref is None else _f64(q_ref)
+        _check(lib().serbf_norms(self._e, int(precision), _p(qr), _p(out)), "norms")
+        return dict(zip(("L1", "L2", "Linf", "mass_ref", "mass", "min", "max"), out))
