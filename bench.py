@@ -89,18 +89,20 @@ def main():
     eng.set_state(q0)
     dt = courant_dt(cfg, N)
 
-    res = {}
+    res, final = {}, {}
     for prec, name in ((F64, "f64"), (F32, "f32")):
         eng.set_state(q0)
         eng.run(prec, args.warmup, dt)
         eng.set_state(q0)
         ms, stage_ms = eng.run_timed(prec, args.steps, dt)
         res[name] = (ms, stage_ms)
+        final[name] = eng.state(prec)
         log(f"{name}: {args.steps} steps {ms:.2f} ms, {ms / args.steps:.3f} ms/step, stage {stage_ms * 1e3:.1f} us")
-    q64 = eng.state(F64)
-    q32 = eng.state(F32)
+    q64, q32 = final["f64"], final["f32"]
     err32 = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
     linf32 = float(np.max(np.abs(q32 - q64)) / np.max(np.abs(q64)))
+    drift = {name: float(np.linalg.norm(q - q0) / np.linalg.norm(q0)) for name, q in (("f64", q64), ("f32", q32))}
+    log(f"f32 vs f64 rel l2 {err32:.3e}, rel linf {linf32:.3e}; change from q0: {drift}")
     pts = N * cfg.Nv * ntr
     v32 = pts * args.steps / (res["f32"][0] / 1e3)
     v64 = pts * args.steps / (res["f64"][0] / 1e3)
@@ -133,7 +135,8 @@ def main():
         "fp64": {"value": v64, "ms_per_step": res["f64"][0] / args.steps, "stage_us": res["f64"][1] * 1e3,
                  "roofline_achieved_GBs": achieved64, "roofline_frac": achieved64 / HBM_PEAK_GBS},
         "speedup_f32_vs_f64": v32 / v64,
-        "error": {"f32_vs_f64_rel_l2": err32, "f32_vs_f64_rel_linf": linf32, "after_steps": args.steps},
+        "error": {"f32_vs_f64_rel_l2": err32, "f32_vs_f64_rel_linf": linf32, "after_steps": args.steps,
+                  "rel_l2_change_from_q0": drift},
         "setup": {"knn_s": knn_s, "weights_s": w_s, "weight_solves_per_s": 2 * N / w_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -159,13 +162,18 @@ def cpu_baseline(eng, cfg, xyz, q0, dt, steps):
     cores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
     t = time.time()
-    P.rk4(q0, steps, dt)
+    q_cpu = P.rk4(q0, steps, dt)
     s = time.time() - t
+    from se_rbf_projects_amd.ctypes_api import F64
+    eng.set_state(q0)
+    eng.run(F64, steps, dt)
+    parity = float(np.linalg.norm(eng.state(F64) - q_cpu) / np.linalg.norm(q_cpu))
+    log(f"full-scale parity, GPU FP64 vs oracle after {steps} steps: rel l2 {parity:.3e}")
     pts = len(xyz) * cfg.Nv * max(1, cfg.ntracers)
     log(f"cpu baseline: {steps} steps in {s:.2f}s on {cores} cores")
     return {"value": pts * steps / s, "unit": "point-updates/s", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
             "kind": "port", "sample": f"{steps} RK4 steps of the full workload (FP64 oracle, OpenMP over nodes)",
-            "cpu": platform.processor() or platform.machine()}
+            "cpu": platform.processor() or platform.machine(), "gpu_f64_vs_oracle_rel_l2": parity}
 
 
 if __name__ == "__main__":

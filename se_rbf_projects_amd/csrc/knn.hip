@@ -53,17 +53,20 @@ __global__ void knn_scatter(int N, const double *x, const double *y, const doubl
     sorted[pos] = make_double4(x[i], y[i], z[i], __longlong_as_double((long long)i));
 }
 
+// Every product and sum rounded separately: no FMA contraction in this function.
 __device__ __forceinline__ double exact_d2(double xi, double yi, double zi, double xj, double yj, double zj)
 {
-    const double dx = __dadd_rn(xi, -xj), dy = __dadd_rn(yi, -yj), dz = __dadd_rn(zi, -zj);
-    double d = __dmul_rn(dx, dx);
-    d = __dadd_rn(d, __dmul_rn(dy, dy));
-    d = __dadd_rn(d, __dmul_rn(dz, dz));
+#pragma clang fp contract(off)
+    const double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+    double d = dx * dx;
+    d = d + dy * dy;
+    d = d + dz * dz;
     return d;
 }
 
 __global__ void __launch_bounds__(64 * kKnnWaves)
-knn_query(int N, int n, const double *x, const double *y, const double *z, int G, double s, const int *start,
+knn_query(int N, int n, const double *x, const double *y, const double *z, int G, double s, double inv_s,
+          const int *start,
           const double4 *sorted, int *out_idx, double *out_D, int *flag)
 {
     __shared__ double kd[kKnnWaves][kKnnCap];
@@ -72,7 +75,6 @@ knn_query(int N, int n, const double *x, const double *y, const double *z, int G
     const int q = __builtin_amdgcn_readfirstlane(blockIdx.x * kKnnWaves + w);
     if (q >= N) return;
     const double xq = x[q], yq = y[q], zq = z[q];
-    const double inv_s = 1.0 / s;
     const int cx = cell_coord(xq, inv_s, G), cy = cell_coord(yq, inv_s, G), cz = cell_coord(zq, inv_s, G);
     double *D2 = kd[w];
     int *ID = ki[w];
@@ -175,7 +177,7 @@ int knn_run(serbf_engine *e, int n)
     SERBF_HIP(hipGetLastError());
     const int qb = (N + kKnnWaves - 1) / kKnnWaves;
     hipLaunchKernelGGL(knn_query, dim3(qb), dim3(64 * kKnnWaves), 0, e->stream, N, n, e->d_x, e->d_y, e->d_z, G, s,
-                       start, sorted, e->d_idx, e->d_D, flag);
+                       1.0 / s, start, sorted, e->d_idx, e->d_D, flag);
     SERBF_HIP(hipGetLastError());
     int hflag = 0;
     SERBF_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, e->stream));

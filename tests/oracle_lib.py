@@ -48,6 +48,12 @@ def lib():
         _lib.orc_ref_eval_sumpartSVpartSV.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                                       C.c_void_p]
         _lib.orc_ref_set_TC2_U_t.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+        _lib.orc_ref_get_stencils.argtypes = [C.c_void_p, C.c_int]
+        _lib.orc_ref_init_HH.argtypes = [C.c_void_p]
+        _lib.orc_ref_update_Up.argtypes = [C.c_void_p]
+        _lib.orc_ref_init_TC2.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.orc_ref_get_solution.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.orc_ref_verify.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     return _lib
 
 

@@ -179,6 +179,31 @@ def test_rk4_four_tracers(eng, golden):
     P = O.Problem(cfg, xyz, idx, Dx, Dy, L)
     oq = P.rk4(q0, 12, dt)
     assert rel_l2(q, oq) < 1e-11
+    eng.run(F32, 12, dt)
+    assert rel_l2(eng.state(F32), oq) < 1e-4
+
+
+def test_rk4_deform_f32_at_level5(eng):
+    """Deformational flow, 10242 nodes x 32 levels, GPU stencils and weights: FP64
+    matches the oracle on the same stencils/weights, FP32 stays within 1e-4."""
+    xyz = E.icos_nodes(5)
+    xyz = xyz[E.spatial_order(xyz)]
+    cfg = config_deform(len(xyz), Nv=32, n=55, deg=4, nv=7)
+    eng.set_nodes(xyz)
+    eng.knn(cfg.n)
+    eng.configure(cfg)
+    eng.weights()
+    idx, _ = eng.stencils()
+    Dx, Dy, L, _ = eng.get_weights()
+    q0 = O.initial(E.IC_HILLS, cfg, xyz, seed=1)
+    eng.set_state(q0)
+    dt = courant_dt(cfg, len(xyz))
+    eng.run(F64, 10, dt)
+    q = eng.state(F64)
+    oq = O.Problem(cfg, xyz, idx, Dx, Dy, L).rk4(q0, 10, dt)
+    assert rel_l2(q, oq) < 1e-11
+    eng.run(F32, 10, dt)
+    assert rel_l2(eng.state(F32), oq) < 1e-4
 
 
 def test_weights_then_rk4_end_to_end(eng):
