@@ -54,6 +54,7 @@ def main():
     ap.add_argument("--tracers", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2, help="RK4 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=1024, help="size of the stream-copy bandwidth reference")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not part of "
@@ -90,7 +91,8 @@ def main():
     dt = courant_dt(cfg, N)
 
     res, final = {}, {}
-    for prec, name in ((F64, "f64"), (F32, "f32")):
+    for prec, name, split in ((F64, "f64", False), (F32, "f32", True)):
+        eng.set_hv_split(split)
         eng.set_state(q0)
         eng.run(prec, args.warmup, dt)
         eng.set_state(q0)
@@ -110,6 +112,10 @@ def main():
     achieved = b32 / (res["f32"][1] / 1e3) / 1e9
     b64 = stage_bytes(N, cfg.n, cfg.Nv, ntr, 8, 8)
     achieved64 = b64 / (res["f64"][1] / 1e3) / 1e9
+
+    copy_ms = eng.stream_copy(args.copy_mb << 20, reps=10)
+    copy_gbs = 2 * (args.copy_mb << 20) / (copy_ms / 1e3) / 1e9
+    log(f"stream copy {args.copy_mb} MiB: {copy_ms:.3f} ms, {copy_gbs:.0f} GB/s (read + write)")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -137,6 +143,7 @@ def main():
         "speedup_f32_vs_f64": v32 / v64,
         "error": {"f32_vs_f64_rel_l2": err32, "f32_vs_f64_rel_linf": linf32, "after_steps": args.steps,
                   "rel_l2_change_from_q0": drift},
+        "stream_copy_GBs": copy_gbs,
         "setup": {"knn_s": knn_s, "weights_s": w_s, "weight_solves_per_s": 2 * N / w_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -165,6 +172,7 @@ def cpu_baseline(eng, cfg, xyz, q0, dt, steps):
     q_cpu = P.rk4(q0, steps, dt)
     s = time.time() - t
     from se_rbf_projects_amd.ctypes_api import F64
+    eng.set_hv_split(False)
     eng.set_state(q0)
     eng.run(F64, steps, dt)
     parity = float(np.linalg.norm(eng.state(F64) - q_cpu) / np.linalg.norm(q_cpu))

@@ -147,6 +147,23 @@ int serbf_run(serbf_engine *e, int precision, int nsteps, double dt, double t0);
  * nsteps steps, *stage_ms = average duration of one stage kernel launch. */
 int serbf_run_timed(serbf_engine *e, int precision, int nsteps, double dt, double t0,
                     double *ms, double *stage_ms);
+/* Hyperviscosity placement: 0 (default) applies hv_coef * Delta^k q inside every RK4
+ * stage (the right-hand side form); 1 applies it once per step as dt c Delta^k q(t)
+ * from the step-start state (operator splitting), which removes the Delta^k stream
+ * from stages 2-4. */
+int serbf_set_hv_split(serbf_engine *e, int split);
+/* Per-point hyperviscosity coefficients c[N*ntracers*Nv] (1/s) replacing hv_coef, e.g.
+ * from the learned closure; NULL returns to the constant hv_coef. */
+int serbf_set_gamma(serbf_engine *e, const double *gamma);
+/* Stage kernel variant: 2 (default) column-vector form, 1 lane-per-level form. */
+int serbf_set_kernel_version(serbf_engine *e, int version);
+/* Neighbour columns each wavefront loads before consuming them (4, 8 or 16): the
+ * memory-level parallelism / register trade-off of the stage kernel. */
+int serbf_set_stage_unroll(serbf_engine *e, int unroll);
+/* Streams `bytes` from one device buffer to another reps times (16-byte vector
+ * copy kernel); *ms_per_copy is the average launch time.  Reference for the
+ * attainable HBM bandwidth and for calibrating the HBM byte counters. */
+int serbf_stream_copy(serbf_engine *e, size_t bytes, int reps, double *ms_per_copy);
 /* Reset the FP32 state from the FP64 state. */
 int serbf_sync_f32_from_f64(serbf_engine *e);
 /* Relative norms of state(precision) against q_ref (host, N*Nv):

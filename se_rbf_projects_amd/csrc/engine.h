@@ -103,6 +103,12 @@ struct serbf_engine {
     int *d_base = nullptr;    // step counter read by the stage kernels
     // closure coefficient mode: 0 none/constant, 1 per-point gamma
     int hv_mode = 0;
+    // hyperviscosity once per step (1) or in every stage (0)
+    int hv_split = 0;
+    // stage kernel: 2 column-vector form, 1 lane-per-level form
+    int kver = 2;
+    // neighbour loads issued together per wavefront in the stage kernel (4, 8, 16)
+    int unroll = 8;
     serbf::GraphCache graph;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // reusable scratch

@@ -55,6 +55,14 @@ __global__ void norms_kernel(size_t ncols, int Nv, int S, const double *qr, cons
         for (int k = 0; k < 10; k++) part[blockIdx.x * 10 + k] = red[k][0];
 }
 
+// Plain streaming copy (16 B per lane per iteration): the attainable-bandwidth and
+// HBM-counter calibration reference.
+__global__ void stream_copy_kernel(size_t n16, const float4 *src, float4 *dst)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace serbf
 
 using namespace serbf;
@@ -215,7 +223,7 @@ int serbf_configure(serbf_engine *e, const serbf_config *cfg)
     e->have_state = false;
     e->cfg = c;
     e->ntr = ntr;
-    e->S = c.Nv <= 16 ? 16 : (c.Nv <= 32 ? 32 : 64);
+    e->S = c.Nv <= 16 ? 16 : 64;
     const int nv = vertical_table(c, e->h_vstart, e->h_vW);
     SERBF_CHECK(nv > 0, SERBF_ERR_SINGULAR, "configure: vertical stencil system is singular");
     e->nv = nv;
@@ -374,6 +382,74 @@ int serbf_run_timed(serbf_engine *e, int precision, int nsteps, double dt, doubl
     SERBF_CHECK(e && ms && stage_ms, SERBF_ERR_ARG, "run_timed: null");
     SERBF_HIP(hipSetDevice(e->device));
     return rk4_run(e, precision, nsteps, dt, t0, true, ms, stage_ms);
+}
+
+int serbf_stream_copy(serbf_engine *e, size_t bytes, int reps, double *ms_per_copy)
+{
+    SERBF_CHECK(e && bytes >= 16 && reps >= 1 && ms_per_copy, SERBF_ERR_ARG, "stream_copy: bad arguments");
+    SERBF_HIP(hipSetDevice(e->device));
+    const size_t n16 = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    SERBF_HIP(hipMalloc((void **)&a, n16 * 16));
+    SERBF_HIP(hipMalloc((void **)&b, n16 * 16));
+    SERBF_HIP(hipMemsetAsync(a, 0, n16 * 16, e->stream));
+    const int blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, e->stream, n16, a, b);
+    SERBF_HIP(hipEventRecord(e->ev0, e->stream));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, e->stream, n16, a, b);
+    SERBF_HIP(hipEventRecord(e->ev1, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    float t = 0.f;
+    SERBF_HIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
+    *ms_per_copy = t / reps;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return SERBF_OK;
+}
+
+int serbf_set_hv_split(serbf_engine *e, int split)
+{
+    SERBF_CHECK(e, SERBF_ERR_ARG, "set_hv_split: null");
+    e->hv_split = split ? 1 : 0;
+    return SERBF_OK;
+}
+
+int serbf_set_kernel_version(serbf_engine *e, int version)
+{
+    SERBF_CHECK(e && (version == 1 || version == 2), SERBF_ERR_ARG, "set_kernel_version: 1 or 2");
+    e->kver = version;
+    return SERBF_OK;
+}
+
+int serbf_set_stage_unroll(serbf_engine *e, int unroll)
+{
+    SERBF_CHECK(e && (unroll == 4 || unroll == 8 || unroll == 16), SERBF_ERR_ARG, "set_stage_unroll: 4, 8 or 16");
+    e->unroll = unroll;
+    return SERBF_OK;
+}
+
+int serbf_set_gamma(serbf_engine *e, const double *gamma)
+{
+    SERBF_CHECK(e && e->have_state, SERBF_ERR_STATE, "set_gamma: state first");
+    if (!gamma) {
+        e->hv_mode = 0;
+        return SERBF_OK;
+    }
+    const int Nv = e->cfg.Nv, S = e->S;
+    const size_t cols = (size_t)e->N * e->ntr, P = cols * S;
+    if (!e->s64.gamma) {
+        SERBF_HIP(hipMalloc((void **)&e->s64.gamma, P * sizeof(double)));
+        SERBF_HIP(hipMalloc((void **)&e->s32.gamma, P * sizeof(float)));
+        SERBF_HIP(hipMemsetAsync(e->s64.gamma, 0, P * sizeof(double), e->stream));
+    }
+    SERBF_HIP(hipMemcpy2DAsync(e->s64.gamma, S * sizeof(double), gamma, Nv * sizeof(double), Nv * sizeof(double), cols,
+                               hipMemcpyHostToDevice, e->stream));
+    int rc = convert_to_f32(e, P, e->s64.gamma, e->s32.gamma);
+    if (rc) return rc;
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    e->hv_mode = 1;
+    return SERBF_OK;
 }
 
 int serbf_norms(serbf_engine *e, int precision, const double *q_ref, double *out)

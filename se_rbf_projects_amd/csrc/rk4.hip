@@ -204,6 +204,324 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs<T> a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Stage kernel, column-vector form.
+//
+// A column is one (node, tracer) pair of S levels; LPC = S/4 lanes hold it, four
+// consecutive levels per lane, so one 16-byte (FP32) or 32-byte (FP64) load per
+// lane fetches a neighbour's whole column and, with four tracers, one wavefront
+// instruction fetches all tracers of a neighbour (1 KB FP32).  Columns per
+// wavefront CPW = 64/LPC; with four tracers the CPW columns are the tracers of
+// CPW/4 nodes.  Per node, the stencil's indices and weights are staged in LDS
+// once per launch; when the horizontal velocity does not depend on the level
+// (solid body, deformational flow in config mode) the velocity is folded into
+// one advection weight wc_s = Up_x Dx_s + Up_y Dy_s, so the inner loop carries
+// two multiply-add streams (advection and Delta^k) instead of three.  The
+// vertical stencil reads the node's own column back from LDS, with 8 zero
+// levels on either side (the reference's zero ghosts).
+//
+// Hyperviscosity placement (HV): 0 none; 1 in every stage (the right-hand side
+// F = -U.grad q + c L q of the reference form); 2 split: applied once per step as
+// dt c L q_T, folded into stage 1's accumulator (F = f1 + 6 c L q_T), so stages
+// 2-4 carry no Delta^k stream.
+
+template <typename T>
+struct Vec4 {
+    T v[4];
+};
+
+template <typename T>
+__device__ __forceinline__ Vec4<T> load4(const T *p)
+{
+    Vec4<T> r;
+    if constexpr (sizeof(T) == 4) {
+        const float4 x = *(const float4 *)p;
+        r.v[0] = x.x; r.v[1] = x.y; r.v[2] = x.z; r.v[3] = x.w;
+    } else {
+        const double2 x = *(const double2 *)p, y = *(const double2 *)(p + 2);
+        r.v[0] = x.x; r.v[1] = x.y; r.v[2] = y.x; r.v[3] = y.y;
+    }
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void store4(T *p, const Vec4<T> &r)
+{
+    if constexpr (sizeof(T) == 4) {
+        *(float4 *)p = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+    } else {
+        *(double2 *)p = make_double2(r.v[0], r.v[1]);
+        *(double2 *)(p + 2) = make_double2(r.v[2], r.v[3]);
+    }
+}
+
+// stencil entry in LDS: (weight a, weight b, Delta^k weight, neighbour index)
+template <typename T>
+struct alignas(4 * sizeof(T)) MetaE {
+    T a, b, l, idx_bits;
+};
+
+template <typename T>
+__device__ __forceinline__ T idx_to_bits(int i)
+{
+    if constexpr (sizeof(T) == 4) return __int_as_float(i);
+    else return __longlong_as_double((long long)i);
+}
+template <typename T>
+__device__ __forceinline__ int bits_to_idx(T b)
+{
+    if constexpr (sizeof(T) == 4) return __float_as_int(b);
+    else return (int)__double_as_longlong(b);
+}
+
+// horizontal velocity of a node at a level-independent flow (config mode)
+template <typename T>
+__device__ __forceinline__ void node_velocity(const StageArgs<T> &a, const T *B, const T *ts, T &upx, T &upy)
+{
+    if (a.flow == VEL_DEFORM) {
+        const T c = ts[0], C2 = ts[1], S2 = ts[2];
+        const T cl = T(0.5) - T(0.5) * (B[9] * C2 + B[10] * S2);
+        const T cp = B[10] * C2 - B[9] * S2;
+        upx = B[0] + c * (cl * B[3] + cp * B[6]);
+        upy = B[1] + c * (cl * B[4] + cp * B[7]);
+    } else {
+        upx = B[0];
+        upy = B[1];
+    }
+}
+
+// velocity at level v (any flow)
+template <typename T, int S>
+__device__ __forceinline__ void level_velocity(const StageArgs<T> &a, const T *B, const T *ts, int pc, int v, T &upx,
+                                               T &upy, T &upz)
+{
+    if (a.flow == VEL_TC2) {
+        const T wt = ts[0];
+        const T ga = wt * a.lev[S + v], gb = wt * a.lev[2 * S + v];
+        upx = B[0] + (ga * B[3] + gb * B[6]);
+        upy = B[1] + (ga * B[4] + gb * B[7]);
+        upz = a.ref_mode ? B[2] + (ga * B[5] + gb * B[8]) : gb * B[9];
+    } else if (a.flow == VEL_DEFORM) {
+        const T c = ts[0], C2 = ts[1], S2 = ts[2];
+        const T cl = T(0.5) - T(0.5) * (B[9] * C2 + B[10] * S2);
+        const T cp = B[10] * C2 - B[9] * S2;
+        const T w = c * a.lev[S + v];
+        upx = B[0] + c * (cl * B[3] + cp * B[6]) + w * B[11];
+        upy = B[1] + c * (cl * B[4] + cp * B[7]) + w * B[12];
+        upz = a.ref_mode ? B[2] + c * (cl * B[5] + cp * B[8]) + w * B[13] : w;
+    } else if (a.flow == VEL_SOLID) {
+        upx = B[0];
+        upy = B[1];
+        upz = a.ref_mode ? B[2] : T(0);
+    } else {
+        const T *u = a.uarr + (size_t)pc * 3 * S + v;
+        upx = u[0];
+        upy = u[S];
+        upz = u[2 * S];
+    }
+}
+
+template <typename T, int S, int NTR>
+struct ColLayout {
+    static constexpr int LPC = S / 4;            // lanes per column
+    static constexpr int CPW = 64 / LPC;         // columns per wavefront
+    static constexpr int NPW = CPW / NTR;        // nodes per wavefront
+    static constexpr int COLW = S + 16;          // LDS column with 8 zero levels each side
+    static constexpr int WAVES = 2;              // wavefronts per work-group
+    __host__ __device__ static size_t wave_bytes(int npad) { return (size_t)NPW * npad * sizeof(MetaE<T>) + (size_t)CPW * COLW * sizeof(T); }
+};
+
+template <typename T, int S, int NTR, int STAGE, int HV, int U>
+__global__ void __launch_bounds__(128) stage2_kernel(StageArgs<T> a, int npad, int xy)
+{
+    using L = ColLayout<T, S, NTR>;
+    constexpr int LPC = L::LPC, CPW = L::CPW, NPW = L::NPW, COLW = L::COLW;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ T vtab[S * kMaxNv];
+    __shared__ int vst[S];
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int blk = xcd_remap(blockIdx.x, a.nblk);
+    const int wave = blk * L::WAVES + w;
+    const int col = lane / LPC, lc = lane % LPC;
+    const int slot = col / NTR, tr = col % NTR;
+    const int n = a.n;
+
+    // per-level vertical stencils of the block
+    for (int i = threadIdx.x; i < S * kMaxNv; i += blockDim.x) vtab[i] = a.vW[i];
+    for (int i = threadIdx.x; i < S; i += blockDim.x) vst[i] = a.vstart[i];
+    __syncthreads();
+
+    const int node0 = wave * NPW;
+    if (node0 >= a.N) return;
+    unsigned char *wbase = lds_raw + (size_t)w * L::wave_bytes(npad);
+    MetaE<T> *meta = (MetaE<T> *)wbase;
+    T *colbuf = (T *)(wbase + (size_t)NPW * npad * sizeof(MetaE<T>));
+
+    const T *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
+
+    // ---- stage the stencils of the wavefront's nodes in LDS ----
+    for (int j = lane; j < NPW * npad; j += 64) {
+        const int sl = j / npad, s = j - sl * npad;
+        int p = node0 + sl;
+        if (p >= a.N) p = a.N - 1;
+        MetaE<T> m;
+        if (s < n) {
+            const size_t o = (size_t)p * n + s;
+            const T wx = a.Wx[o], wy = a.Wy[o];
+            if (xy) {
+                m.a = wx;
+                m.b = wy;
+            } else {
+                T ux, uy;
+                node_velocity<T>(a, a.basis + (size_t)p * kBasis, ts, ux, uy);
+                m.a = ux * wx + uy * wy;
+                m.b = T(0);
+            }
+            m.l = HV ? a.WL[o] : T(0);
+            m.idx_bits = idx_to_bits<T>(a.idx[o]);
+        } else {
+            m.a = T(0); m.b = T(0); m.l = T(0);
+            m.idx_bits = idx_to_bits<T>(p);
+        }
+        meta[j] = m;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+
+    int p = node0 + slot;
+    const bool node_ok = p < a.N;
+    if (!node_ok) p = a.N - 1;
+    const size_t my = ((size_t)p * NTR + tr) * S + 4 * lc;   // this lane's 4 levels
+
+    // ---- horizontal stencil ----
+    T acc_a[4] = {T(0), T(0), T(0), T(0)}, acc_b[4] = {T(0), T(0), T(0), T(0)}, acc_l[4] = {T(0), T(0), T(0), T(0)};
+    const MetaE<T> *mrow = meta + slot * npad;
+    const T *qbase = a.qin + (size_t)tr * S + 4 * lc;
+    if (xy) {
+        for (int s0 = 0; s0 < npad; s0 += U) {
+            MetaE<T> m[U];
+            Vec4<T> q[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) m[u] = mrow[s0 + u];
+#pragma unroll
+            for (int u = 0; u < U; u++) q[u] = load4<T>(qbase + (size_t)bits_to_idx<T>(m[u].idx_bits) * NTR * S);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    acc_a[k] = fma(m[u].a, q[u].v[k], acc_a[k]);
+                    acc_b[k] = fma(m[u].b, q[u].v[k], acc_b[k]);
+                    if (HV) acc_l[k] = fma(m[u].l, q[u].v[k], acc_l[k]);
+                }
+        }
+    } else {
+        for (int s0 = 0; s0 < npad; s0 += U) {
+            MetaE<T> m[U];
+            Vec4<T> q[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) m[u] = mrow[s0 + u];
+#pragma unroll
+            for (int u = 0; u < U; u++) q[u] = load4<T>(qbase + (size_t)bits_to_idx<T>(m[u].idx_bits) * NTR * S);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    acc_a[k] = fma(m[u].a, q[u].v[k], acc_a[k]);
+                    if (HV) acc_l[k] = fma(m[u].l, q[u].v[k], acc_l[k]);
+                }
+        }
+    }
+
+    // ---- own column: vertical stencil through LDS ----
+    const Vec4<T> self = load4<T>(a.qin + my);
+    T *cb = colbuf + col * COLW;
+#pragma unroll
+    for (int k = 0; k < 4; k++) cb[8 + 4 * lc + k] = self.v[k];
+    if (lc == 0)
+        for (int k = 0; k < 8; k++) cb[k] = T(0);
+    if (lc == LPC - 1)
+        for (int k = 0; k < 8; k++) cb[8 + S + k] = T(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    T az[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int v = 4 * lc + k;
+        T z = T(0);
+        if (v < a.Nv) {
+            const int st = vst[v];
+            const T *wv = vtab + v * kMaxNv;
+            for (int j = 0; j < a.nv; j++) z = fma(wv[j], cb[8 + st + j], z);
+        }
+        az[k] = z;
+    }
+
+    // ---- right-hand side and the fused RK4 update ----
+    const T *B = a.basis + (size_t)p * kBasis;
+    T coef[4];
+    if (HV) {
+        if (a.hv_mode) {
+            const Vec4<T> g = load4<T>(a.gamma + my);
+#pragma unroll
+            for (int k = 0; k < 4; k++) coef[k] = g.v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) coef[k] = a.hv_coef;
+        }
+    }
+    Vec4<T> f, fh;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int v = 4 * lc + k;
+        const T r_inv = a.lev[v];
+        T upx, upy, upz;
+        level_velocity<T, S>(a, B, ts, p, v, upx, upy, upz);
+        T adv;
+        if (xy) adv = upx * (acc_a[k] * r_inv) + upy * (acc_b[k] * r_inv);
+        else adv = acc_a[k] * r_inv;
+        T fv = a.alpha * (adv + upz * az[k]);
+        T hv = T(0);
+        if (HV == 1) fv += coef[k] * acc_l[k];
+        if (HV == 2) hv = T(6) * coef[k] * acc_l[k];
+        const bool live = v < a.Nv;
+        f.v[k] = live ? fv : T(0);
+        fh.v[k] = live ? hv : T(0);
+    }
+    if (!node_ok) return;
+    Vec4<T> o;
+    if (STAGE == 0) {
+        store4<T>(a.F + my, f);
+    } else if (STAGE == 1) {
+        Vec4<T> F1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            F1.v[k] = f.v[k] + fh.v[k];
+            o.v[k] = 4 * lc + k < a.Nv ? self.v[k] + a.hdt * f.v[k] : T(0);
+        }
+        store4<T>(a.F + my, F1);
+        store4<T>(a.qout + my, o);
+    } else if (STAGE == 2 || STAGE == 3) {
+        const Vec4<T> Fo = load4<T>(a.F + my), qt = load4<T>(a.qT + my);
+        Vec4<T> Fn;
+        const T h = STAGE == 2 ? a.hdt : a.dt;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            Fn.v[k] = Fo.v[k] + T(2) * f.v[k];
+            o.v[k] = 4 * lc + k < a.Nv ? qt.v[k] + h * f.v[k] : T(0);
+        }
+        store4<T>(a.F + my, Fn);
+        store4<T>(a.qout + my, o);
+    } else {
+        const Vec4<T> Fo = load4<T>(a.F + my), qt = load4<T>(a.qT + my);
+#pragma unroll
+        for (int k = 0; k < 4; k++) o.v[k] = 4 * lc + k < a.Nv ? qt.v[k] + a.dt6 * (Fo.v[k] + T(1) * f.v[k]) : T(0);
+        store4<T>(a.qout + my, o);
+    }
+}
+
 __global__ void bump_kernel(int *base, int by)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0)
@@ -451,13 +769,69 @@ void launch_lanes(serbf_engine *e, int stage, bool hv, StageArgs<T> a)
     }
 }
 
-template <typename T>
-int launch_stage(serbf_engine *e, int stage, bool hv, const StageArgs<T> &a)
+template <typename T, int S, int NTR, int U>
+void launch2u(serbf_engine *e, int stage, int hv, bool xy, StageArgs<T> a)
 {
-    switch (e->S) {
-    case 16: launch_lanes<T, 16>(e, stage, hv, a); break;
-    case 32: launch_lanes<T, 32>(e, stage, hv, a); break;
-    default: launch_lanes<T, 64>(e, stage, hv, a); break;
+    using L = ColLayout<T, S, NTR>;
+    const int npad = (e->n + U - 1) / U * U;
+    const int nwaves = (e->N + L::NPW - 1) / L::NPW;
+    const int nblk = (nwaves + L::WAVES - 1) / L::WAVES;
+    a.nblk = nblk;
+    const size_t lds = L::WAVES * L::wave_bytes(npad);
+    const dim3 grid(nblk), block(64 * L::WAVES);
+    const int x = xy ? 1 : 0;
+#define SERBF_L2(ST, HVV)                                                                                      \
+    do {                                                                                                     \
+        if (lds > 48 * 1024)                                                                                 \
+            (void)hipFuncSetAttribute((const void *)stage2_kernel<T, S, NTR, ST, HVV, U>,                    \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                 \
+        hipLaunchKernelGGL((stage2_kernel<T, S, NTR, ST, HVV, U>), grid, block, lds, e->stream, a, npad, x); \
+    } while (0)
+    switch (stage * 3 + hv) {
+    case 0: SERBF_L2(0, 0); break;
+    case 1: SERBF_L2(0, 1); break;
+    case 3: SERBF_L2(1, 0); break;
+    case 4: SERBF_L2(1, 1); break;
+    case 5: SERBF_L2(1, 2); break;
+    case 6: SERBF_L2(2, 0); break;
+    case 7: SERBF_L2(2, 1); break;
+    case 9: SERBF_L2(3, 0); break;
+    case 10: SERBF_L2(3, 1); break;
+    case 12: SERBF_L2(4, 0); break;
+    default: SERBF_L2(4, 1); break;
+    }
+#undef SERBF_L2
+}
+
+template <typename T, int S, int NTR>
+void launch2(serbf_engine *e, int stage, int hv, bool xy, StageArgs<T> a)
+{
+    switch (e->unroll) {
+    case 8: launch2u<T, S, NTR, 8>(e, stage, hv, xy, a); break;
+    case 16: launch2u<T, S, NTR, 16>(e, stage, hv, xy, a); break;
+    default: launch2u<T, S, NTR, 4>(e, stage, hv, xy, a); break;
+    }
+}
+
+// hv: 0 none, 1 hyperviscosity in this stage, 2 split (stage 1 only)
+template <typename T>
+int launch_stage(serbf_engine *e, int stage, int hv, const StageArgs<T> &a)
+{
+    if (e->kver == 1) {
+        switch (e->S) {
+        case 16: launch_lanes<T, 16>(e, stage, hv != 0, a); break;
+        case 32: launch_lanes<T, 32>(e, stage, hv != 0, a); break;
+        default: launch_lanes<T, 64>(e, stage, hv != 0, a); break;
+        }
+    } else {
+        const bool xy = a.flow == VEL_TC2 || a.flow == VEL_ARRAY || (a.ref_mode && a.flow == VEL_DEFORM);
+        if (e->S == 16) {
+            if (e->ntr == 1) launch2<T, 16, 1>(e, stage, hv, xy, a);
+            else launch2<T, 16, 4>(e, stage, hv, xy, a);
+        } else {
+            if (e->ntr == 1) launch2<T, 64, 1>(e, stage, hv, xy, a);
+            else launch2<T, 64, 4>(e, stage, hv, xy, a);
+        }
     }
     SERBF_HIP(hipGetLastError());
     return SERBF_OK;
@@ -495,20 +869,21 @@ bool hv_enabled(serbf_engine *e)
 template <typename T>
 int enqueue_step(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off)
 {
-    const bool hv = hv_enabled(e);
+    const int hv = hv_enabled(e) ? 1 : 0;
+    const int hv1 = hv && e->hv_split ? 2 : hv, hvr = hv && e->hv_split ? 0 : hv;
     StageArgs<T> a = base_args<T>(e, sb, dt);
     a.step_off = step_off;
     a.qT = sb.qT;
     a.F = sb.F;
     int rc;
     a.qin = sb.qT; a.qout = sb.qB;
-    if ((rc = launch_stage<T>(e, 1, hv, a))) return rc;
+    if ((rc = launch_stage<T>(e, 1, hv1, a))) return rc;
     a.qin = sb.qB; a.qout = sb.qA;
-    if ((rc = launch_stage<T>(e, 2, hv, a))) return rc;
+    if ((rc = launch_stage<T>(e, 2, hvr, a))) return rc;
     a.qin = sb.qA; a.qout = sb.qB;
-    if ((rc = launch_stage<T>(e, 3, hv, a))) return rc;
+    if ((rc = launch_stage<T>(e, 3, hvr, a))) return rc;
     a.qin = sb.qB; a.qout = sb.qT;
-    return launch_stage<T>(e, 4, hv, a);
+    return launch_stage<T>(e, 4, hvr, a);
 }
 
 constexpr int kGraphSteps = 8;
@@ -523,7 +898,7 @@ int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, doubl
     SERBF_HIP(hipMemcpyAsync(e->d_base, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
     // the graph of kGraphSteps steps reads the step counter and bumps it at the end
     const void *key = (const void *)sb.tscal;
-    const int hvk = (hv_enabled(e) ? 1 : 0) + 2 * e->hv_mode;
+    const int hvk = (hv_enabled(e) ? 1 : 0) + 2 * e->hv_mode + 4 * e->hv_split + 8 * e->kver + 32 * e->unroll;
     if (nsteps >= kGraphSteps && (!e->graph.exec || e->graph.precision != precision || e->graph.key != key ||
                                   e->graph.dt != dt || e->graph.hv != hvk)) {
         if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
@@ -613,7 +988,7 @@ int rk4_rhs(serbf_engine *e, double t, double *F_host)
     a.base = e->d_base + 1;   // always zero
     a.qin = e->s64.qT;
     a.F = e->s64.qA;
-    int rc = launch_stage<double>(e, 0, hv_enabled(e), a);
+    int rc = launch_stage<double>(e, 0, hv_enabled(e) ? 1 : 0, a);
     if (!rc) {
         const size_t P = (size_t)e->N * e->ntr;
         SERBF_HIP(hipMemcpy2DAsync(F_host, e->cfg.Nv * sizeof(double), e->s64.qA, e->S * sizeof(double),
@@ -649,7 +1024,7 @@ int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha
     a.alpha = alpha;
     a.qin = e->s64.qB;
     a.F = e->s64.qA;
-    int rc = launch_stage<double>(e, 0, false, a);
+    int rc = launch_stage<double>(e, 0, 0, a);
     if (!rc) {
         SERBF_HIP(hipMemcpy2DAsync(F_host, Nv * sizeof(double), e->s64.qA, S * sizeof(double), Nv * sizeof(double),
                                    (size_t)N, hipMemcpyDeviceToHost, e->stream));

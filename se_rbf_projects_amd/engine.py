@@ -62,6 +62,11 @@ def lib():
             "serbf_run_timed": ([vp, i, i, d, d, vp, vp], i),
             "serbf_sync_f32_from_f64": ([vp], i),
             "serbf_norms": ([vp, i, vp, vp], i),
+            "serbf_set_hv_split": ([vp, i], i),
+            "serbf_set_gamma": ([vp, vp], i),
+            "serbf_set_kernel_version": ([vp, i], i),
+            "serbf_stream_copy": ([vp, C.c_size_t, i, vp], i),
+            "serbf_set_stage_unroll": ([vp, i], i),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -217,6 +222,24 @@ class Engine:
         _check(lib().serbf_run_timed(self._e, int(precision), int(nsteps), float(dt), float(t0), C.byref(ms),
                                      C.byref(st)), "run_timed")
         return ms.value, st.value
+
+    def set_hv_split(self, split):
+        _check(lib().serbf_set_hv_split(self._e, int(bool(split))), "set_hv_split")
+
+    def set_gamma(self, gamma):
+        g = None if gamma is None else _f64(gamma)
+        _check(lib().serbf_set_gamma(self._e, _p(g)), "set_gamma")
+
+    def set_kernel_version(self, v):
+        _check(lib().serbf_set_kernel_version(self._e, int(v)), "set_kernel_version")
+
+    def set_stage_unroll(self, u):
+        _check(lib().serbf_set_stage_unroll(self._e, int(u)), "set_stage_unroll")
+
+    def stream_copy(self, nbytes, reps=10):
+        ms = C.c_double()
+        _check(lib().serbf_stream_copy(self._e, int(nbytes), int(reps), C.byref(ms)), "stream_copy")
+        return ms.value
 
     def sync_f32(self):
         _check(lib().serbf_sync_f32_from_f64(self._e), "sync_f32")

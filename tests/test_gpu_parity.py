@@ -220,3 +220,76 @@ def test_weights_then_rk4_end_to_end(eng):
     q = eng.state(F64)
     # interior levels see only the horizontal operator, whose weights sum to zero
     assert np.max(np.abs(q - 1.0)) < 1e-9
+
+
+@pytest.mark.parametrize("kver", [1, 2])
+def test_kernel_versions_agree_with_oracle(eng, probA, kver):
+    cfg, xyz, idx, Dx, Dy, L = probA
+    q0 = O.initial(E.IC_BELL, cfg, xyz)
+    dt = courant_dt(cfg, len(xyz))
+    _engine_with(eng, cfg, xyz, idx, Dx, Dy, L, q0)
+    eng.set_kernel_version(kver)
+    try:
+        eng.run(F64, 16, dt)
+        q = eng.state(F64)
+    finally:
+        eng.set_kernel_version(2)
+    oq = O.Problem(cfg, xyz, idx, Dx, Dy, L).rk4(q0, 16, dt)
+    assert rel_l2(q, oq) < 1e-11
+
+
+def _split_oracle(cfg, xyz, idx, Dx, Dy, L, q0, nsteps, dt, gamma=None):
+    """RK4 of the advection alone plus dt * c * (L q) from each step's start state."""
+    import ctypes as C
+
+    cfg0 = make_config(**{k: getattr(cfg, k) for k, _ in cfg._fields_})
+    cfg0.hv_coef = 0.0
+    P0 = O.Problem(cfg0, xyz, idx, Dx, Dy, L)
+    ntr = max(1, cfg.ntracers)
+    q = q0.copy()
+    for s in range(nsteps):
+        qn = P0.rk4(q, 1, dt, t0=s * dt)
+        qq = q.reshape(len(xyz), ntr, cfg.Nv)
+        Lq = np.einsum("ps,pstv->ptv", L, qq[idx]).reshape(q.shape)
+        c = cfg.hv_coef if gamma is None else gamma
+        q = qn + dt * c * Lq
+    return q
+
+
+def test_hv_split_matches_oracle(eng, probA):
+    cfg, xyz, idx, Dx, Dy, L = probA
+    q0 = O.initial(E.IC_BELL, cfg, xyz)
+    dt = courant_dt(cfg, len(xyz))
+    _engine_with(eng, cfg, xyz, idx, Dx, Dy, L, q0)
+    eng.set_hv_split(True)
+    try:
+        eng.run(F64, 10, dt)
+        q = eng.state(F64)
+        eng.run(F32, 10, dt)
+        q32 = eng.state(F32)
+    finally:
+        eng.set_hv_split(False)
+    oq = _split_oracle(cfg, xyz, idx, Dx, Dy, L, q0, 10, dt)
+    assert rel_l2(q, oq) < 1e-11
+    assert rel_l2(q32, oq) < 1e-4
+
+
+def test_per_point_gamma_matches_oracle(eng, probA):
+    cfg, xyz, idx, Dx, Dy, L = probA
+    q0 = O.initial(E.IC_BELL, cfg, xyz)
+    rng = np.random.default_rng(4)
+    gamma = cfg.hv_coef * rng.uniform(0.5, 2.0, size=q0.shape)
+    _engine_with(eng, cfg, xyz, idx, Dx, Dy, L, q0)
+    eng.set_gamma(gamma)
+    try:
+        F = eng.rhs(500.0)
+        dt = courant_dt(cfg, len(xyz))
+        eng.set_hv_split(True)
+        eng.run(F64, 5, dt)
+        q = eng.state(F64)
+    finally:
+        eng.set_gamma(None)
+        eng.set_hv_split(False)
+    P = O.Problem(cfg, xyz, idx, Dx, Dy, L)
+    assert relmax(F, P.rhs(q0, 500.0, gamma=gamma)) < 1e-12
+    assert rel_l2(q, _split_oracle(cfg, xyz, idx, Dx, Dy, L, q0, 5, dt, gamma=gamma)) < 1e-11
