@@ -155,11 +155,16 @@ int serbf_set_hv_split(serbf_engine *e, int split);
 /* Per-point hyperviscosity coefficients c[N*ntracers*Nv] (1/s) replacing hv_coef, e.g.
  * from the learned closure; NULL returns to the constant hv_coef. */
 int serbf_set_gamma(serbf_engine *e, const double *gamma);
-/* Stage kernel variant: 2 (default) column-vector form, 1 lane-per-level form. */
+/* Stage kernel variant: 2 (default) column-vector form, 3 LDS-tiled (columns of 64
+ * levels, 1 or 4 tracers; other shapes use 2), 1 lane-per-level form. */
 int serbf_set_kernel_version(serbf_engine *e, int version);
 /* Neighbour columns each wavefront loads before consuming them (4, 8 or 16): the
  * memory-level parallelism / register trade-off of the stage kernel. */
 int serbf_set_stage_unroll(serbf_engine *e, int unroll);
+/* Bytes of each column chunk the LDS-tiled stage kernel stages per pass (128 or 256),
+ * and the tile plan in use (number of 16-node tiles, largest neighbour union). */
+int serbf_set_tile_chunk(serbf_engine *e, int bytes);
+int serbf_tile_info(serbf_engine *e, int *ntiles, int *umax);
 /* Streams `bytes` from one device buffer to another reps times (16-byte vector
  * copy kernel); *ms_per_copy is the average launch time.  Reference for the
  * attainable HBM bandwidth and for calibrating the HBM byte counters. */

@@ -105,12 +105,19 @@ struct serbf_engine {
     int hv_mode = 0;
     // hyperviscosity once per step (1) or in every stage (0)
     int hv_split = 0;
-    // stage kernel: 2 column-vector form, 1 lane-per-level form
+    // stage kernel: 3 LDS-tiled, 2 column-vector form, 1 lane-per-level form
     int kver = 2;
     // neighbour loads issued together per wavefront in the stage kernel (4, 8, 16)
     int unroll = 8;
     serbf::GraphCache graph;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // LDS tile plan of the stage kernel (16-node tiles): union ids, counts, local indices
+    int *d_tuni = nullptr, *d_tcount = nullptr;
+    unsigned short *d_lidx = nullptr;
+    int tile_ustride = 0, tile_umax = 0, ntiles = 0;
+    bool have_plan = false;
+    bool plan_failed = false;
+    int tile_chunk = 128;   // bytes of each column chunk staged in LDS (128 or 256)
     // reusable scratch
     void *d_scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -128,6 +135,9 @@ int weights_run(serbf_engine *e);
 int vertical_table(const serbf_config &cfg, int *start, double *W);
 // time loop (rk4.hip)
 int rk4_prepare(serbf_engine *e);
+// LDS-tiled stage kernel (tile.hip)
+int tile_plan(serbf_engine *e);
+bool tile_supported(const serbf_engine *e);
 void rk4_free(serbf_engine *e);
 int rk4_rhs(serbf_engine *e, double t, double *F_host);
 int rk4_run(serbf_engine *e, int precision, int nsteps, double dt, double t0, bool timed,

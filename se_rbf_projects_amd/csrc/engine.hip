@@ -94,8 +94,12 @@ static void free_nodes(serbf_engine *e)
     for (double *p : {e->d_x, e->d_y, e->d_z, e->d_lam, e->d_phi, e->d_D, e->d_Wx, e->d_Wy, e->d_WL, e->d_q0,
                       e->d_wmass})
         if (p) (void)hipFree(p);
-    for (int *p : {e->d_idx, e->d_status, e->d_vstart})
+    for (int *p : {e->d_idx, e->d_status, e->d_vstart, e->d_tuni, e->d_tcount})
         if (p) (void)hipFree(p);
+    if (e->d_lidx) (void)hipFree(e->d_lidx);
+    e->d_tuni = e->d_tcount = nullptr;
+    e->d_lidx = nullptr;
+    e->have_plan = false;
     e->d_x = e->d_y = e->d_z = e->d_lam = e->d_phi = e->d_D = e->d_Wx = e->d_Wy = e->d_WL = e->d_q0 =
         e->d_wmass = nullptr;
     e->d_idx = e->d_status = e->d_vstart = nullptr;
@@ -155,6 +159,7 @@ int serbf_knn(serbf_engine *e, int n)
 
 static int alloc_stencils_impl(serbf_engine *e, int n)
 {
+    e->have_plan = false;
     if (e->d_idx && e->n != n) {
         (void)hipFree(e->d_idx);
         (void)hipFree(e->d_D);
@@ -417,8 +422,23 @@ int serbf_set_hv_split(serbf_engine *e, int split)
 
 int serbf_set_kernel_version(serbf_engine *e, int version)
 {
-    SERBF_CHECK(e && (version == 1 || version == 2), SERBF_ERR_ARG, "set_kernel_version: 1 or 2");
+    SERBF_CHECK(e && version >= 1 && version <= 3, SERBF_ERR_ARG, "set_kernel_version: 1, 2 or 3");
     e->kver = version;
+    return SERBF_OK;
+}
+
+int serbf_set_tile_chunk(serbf_engine *e, int bytes)
+{
+    SERBF_CHECK(e && (bytes == 128 || bytes == 256), SERBF_ERR_ARG, "set_tile_chunk: 128 or 256");
+    e->tile_chunk = bytes;
+    return SERBF_OK;
+}
+
+int serbf_tile_info(serbf_engine *e, int *ntiles, int *umax)
+{
+    SERBF_CHECK(e, SERBF_ERR_ARG, "tile_info: null");
+    if (ntiles) *ntiles = e->have_plan ? e->ntiles : 0;
+    if (umax) *umax = e->have_plan ? e->tile_umax : 0;
     return SERBF_OK;
 }
 

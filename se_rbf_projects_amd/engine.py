@@ -67,6 +67,8 @@ def lib():
             "serbf_set_kernel_version": ([vp, i], i),
             "serbf_stream_copy": ([vp, C.c_size_t, i, vp], i),
             "serbf_set_stage_unroll": ([vp, i], i),
+            "serbf_set_tile_chunk": ([vp, i], i),
+            "serbf_tile_info": ([vp, vp, vp], i),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -232,6 +234,14 @@ class Engine:
 
     def set_kernel_version(self, v):
         _check(lib().serbf_set_kernel_version(self._e, int(v)), "set_kernel_version")
+
+    def set_tile_chunk(self, nbytes):
+        _check(lib().serbf_set_tile_chunk(self._e, int(nbytes)), "set_tile_chunk")
+
+    def tile_info(self):
+        nt, um = C.c_int(), C.c_int()
+        _check(lib().serbf_tile_info(self._e, C.byref(nt), C.byref(um)), "tile_info")
+        return nt.value, um.value
 
     def set_stage_unroll(self, u):
         _check(lib().serbf_set_stage_unroll(self._e, int(u)), "set_stage_unroll")
