@@ -1,0 +1,69 @@
+"""Summaries of rocprofv3 runs of the engine.
+
+    python -m se_rbf_projects_amd.profiling trace <dir-with-rocpd-db>
+    python -m se_rbf_projects_amd.profiling pmc <dir> [<dir> ...]
+
+`trace` prints per-kernel call counts and average durations from a
+`rocprofv3 --kernel-trace --stats` run; `pmc` prints per-kernel averages of the
+counters of one or more `rocprofv3 --pmc ...` runs.  Both read the rocpd SQLite
+database rocprofv3 writes (the `kernels` / `counters_collection` views).
+"""
+import collections
+import glob
+import os
+import sqlite3
+import sys
+
+
+def _dbs(path):
+    return sorted(glob.glob(os.path.join(path, "**", "*.db"), recursive=True))
+
+
+def _short(name):
+    return name.split("(")[0].replace("void ", "").replace("serbf::", "")
+
+
+def kernel_stats(path):
+    """{kernel: (calls, average_us, total_us)} from kernel-trace databases under path."""
+    acc = collections.defaultdict(list)
+    for db in _dbs(path):
+        con = sqlite3.connect(db)
+        for name, dur in con.execute("select name, duration from kernels"):
+            acc[_short(name)].append(dur / 1e3)
+    return {k: (len(v), sum(v) / len(v), sum(v)) for k, v in acc.items()}
+
+
+def counter_stats(paths):
+    """{kernel: {counter: average per dispatch}} from pmc databases."""
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in paths:
+        for db in _dbs(path):
+            con = sqlite3.connect(db)
+            for kn, cn, v in con.execute("select kernel_name, counter_name, value from counters_collection"):
+                acc[_short(kn)][cn].append(v)
+    return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
+
+
+def main(argv):
+    if len(argv) < 2:
+        print(__doc__)
+        return 1
+    if argv[0] == "trace":
+        st = kernel_stats(argv[1])
+        tot = sum(v[2] for v in st.values())
+        print("| kernel | calls | average (us) | total (us) | share |")
+        print("|---|---|---|---|---|")
+        for k, (c, a, t) in sorted(st.items(), key=lambda kv: -kv[1][2]):
+            print(f"| `{k}` | {c} | {a:.1f} | {t:.0f} | {100 * t / tot:.1f}% |")
+    else:
+        st = counter_stats(argv[1:])
+        names = sorted({c for d in st.values() for c in d})
+        print("| kernel | " + " | ".join(names) + " |")
+        print("|---" * (len(names) + 1) + "|")
+        for k, d in sorted(st.items()):
+            print(f"| `{k}` | " + " | ".join(f"{d.get(c, float('nan')):.4g}" for c in names) + " |")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
