@@ -8,10 +8,16 @@ and weights are built on the GPU (kNN + batched FP64 weight solves) before the
 timed region; inputs are resident in HBM when it starts.
 
 One step = one classic RK4 step (4 fused stage launches) over the whole state.
-value = FP32 tracer point-updates/s (N * Nv * tracers * steps / s); the FP64
-method on the same inputs is timed beside it.
+One point-update = one (node, level) advanced one RK4 step, all four tracers.
+Timed legs on the same inputs:
+  f64          the plain method in FP64 (hyperviscosity in every stage)
+  f32+closure  FP32 state and SpMV, hyperviscosity once per step with the learned
+               closure's per-point coefficients (re-evaluated every 8 steps)  -> value
+  f32          the same without the closure (its stage kernel gives the roofline)
+--period runs both the f64 and the f32+closure legs over the full 12-day period of the
+reversing flow and states their error against the exact solution q(T) = q(0).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--period]
 """
 import argparse
 import json
@@ -43,24 +49,43 @@ def stage_bytes(N, n, Nv, ntr, bq, bw, basis=16):
     return N * (ntr * Nv * 4.25 * bq + n * (3 * bw + 4) + basis * bw)
 
 
+def load_traffic(kernel_key):
+    """Calibrated HBM bytes per launch of the stage kernel from the committed rocprofv3 counter
+    summary (profiles/stage_kernel_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel_key]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def error_vs(q, qref):
+    d = q[:, :3] - qref[:, :3]
+    return {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(qref[:, :3])),
+            "rel_linf": float(np.max(np.abs(d)) / np.max(np.abs(qref[:, :3])))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--nv", type=int, default=60)
     ap.add_argument("--n", type=int, default=55)
-    ap.add_argument("--tracers", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2, help="RK4 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=1024, help="size of the stream-copy bandwidth reference")
+    ap.add_argument("--period", action="store_true", help="also run the full period and state the errors")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not part of "
               "this build", file=sys.stderr)
         sys.exit(2)
 
+    from se_rbf_projects_amd import closure as CL
     from se_rbf_projects_amd import engine as E
     from se_rbf_projects_amd.ctypes_api import F32, F64, config_deform, courant_dt
 
@@ -68,9 +93,9 @@ def main():
     xyz = E.icos_nodes(args.level)
     xyz = xyz[E.spatial_order(xyz)]
     N = len(xyz)
+    ntr = 4
     cfg = config_deform(N, Nv=args.nv, n=args.n, deg=4, nv=7)
-    cfg.ntracers = args.tracers
-    ntr = args.tracers
+    cfg.ntracers = ntr
     log(f"nodes N={N} generated in {time.time() - t0:.1f}s")
 
     eng = E.Engine(0)
@@ -85,33 +110,65 @@ def main():
     w_s = time.time() - t
     log(f"weights: {w_s:.3f}s ({N / w_s:.3e} nodes/s, 2 systems per node)")
 
-    kinds = [E.IC_HILLS, E.IC_SLOTTED, E.IC_BELLS2, E.IC_CONST][:ntr]
-    q0 = np.stack([E.initial_condition(k, cfg, xyz, seed=2024) for k in kinds], axis=1)
+    q0 = np.stack([E.initial_condition(k, cfg, xyz, seed=2024) for k in CL.TRACERS], axis=1)
     eng.set_state(q0)
-    dt = courant_dt(cfg, N)
+    nsteps_T = int(np.ceil(cfg.T / courant_dt(cfg, N)))
+    dt = cfg.T / nsteps_T
 
+    if os.path.exists(CL.PARAMS_PATH):
+        params, norm, rep = CL.load()
+        closure_kind = "trained"
+    else:
+        params, norm, rep = CL.init_params(0), np.concatenate([np.zeros(16), np.ones(16)]).astype(np.float32), {}
+        closure_kind = "untrained (constant multiplier 1; no closure_params.npz)"
+    closure = E.Closure(0)
+    closure.set_params(params, norm)
+
+    legs = (("f64", F64, False, False), ("f32+closure", F32, True, True), ("f32", F32, True, False))
     res, final = {}, {}
-    for prec, name, split in ((F64, "f64", False), (F32, "f32", True)):
+    for name, prec, split, use_cl in legs:
         eng.set_hv_split(split)
+        eng.attach_closure(closure if use_cl else None, every=8)
         eng.set_state(q0)
         eng.run(prec, args.warmup, dt)
         eng.set_state(q0)
         ms, stage_ms = eng.run_timed(prec, args.steps, dt)
         res[name] = (ms, stage_ms)
         final[name] = eng.state(prec)
-        log(f"{name}: {args.steps} steps {ms:.2f} ms, {ms / args.steps:.3f} ms/step, stage {stage_ms * 1e3:.1f} us")
-    q64, q32 = final["f64"], final["f32"]
-    err32 = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
-    linf32 = float(np.max(np.abs(q32 - q64)) / np.max(np.abs(q64)))
-    drift = {name: float(np.linalg.norm(q - q0) / np.linalg.norm(q0)) for name, q in (("f64", q64), ("f32", q32))}
-    log(f"f32 vs f64 rel l2 {err32:.3e}, rel linf {linf32:.3e}; change from q0: {drift}")
-    pts = N * cfg.Nv * ntr
-    v32 = pts * args.steps / (res["f32"][0] / 1e3)
-    v64 = pts * args.steps / (res["f64"][0] / 1e3)
+        log(f"{name}: {args.steps} steps {ms:.2f} ms, {ms / args.steps:.3f} ms/step, "
+            f"{ms / (4 * args.steps) * 1e3:.1f} us per stage (closure included)")
+    q64 = final["f64"]
+    short = {name: error_vs(final[name], q64) for name in ("f32+closure", "f32")}
+    log(f"after {args.steps} steps, difference from the f64 leg: {short}")
+    pts = N * cfg.Nv
+    value = {name: pts * args.steps / (res[name][0] / 1e3) for name in res}
     b32 = stage_bytes(N, cfg.n, cfg.Nv, ntr, 4, 4)
     achieved = b32 / (res["f32"][1] / 1e3) / 1e9
     b64 = stage_bytes(N, cfg.n, cfg.Nv, ntr, 8, 8)
     achieved64 = b64 / (res["f64"][1] / 1e3) / 1e9
+
+    period = None
+    if args.period:
+        period = {"steps": nsteps_T, "dt_s": dt, "reference": "exact solution q(T) = q(0) of the reversing flow"}
+        for name, prec, split, use_cl in legs[:2]:
+            eng.set_hv_split(split)
+            eng.attach_closure(closure if use_cl else None, every=8)
+            eng.set_state(q0)
+            t = time.time()
+            eng.run(prec, nsteps_T, dt)
+            q = eng.state(prec)
+            period[name] = error_vs(q, q0)
+            period[name]["mass_const_tracer"] = float(np.mean(q[:, 3]) - 1.0)
+            period[name]["seconds"] = time.time() - t
+            log(f"period {name}: {period[name]}")
+        eng.set_hv_split(False)
+        eng.attach_closure(None)
+        eng.set_state(q0)
+        eng.run(F32, nsteps_T, dt)
+        period["f32 plain (unsplit, no closure)"] = error_vs(eng.state(F32), q0)
+        log(f"period f32 plain: {period['f32 plain (unsplit, no closure)']}")
+    eng.attach_closure(None)
+    eng.set_hv_split(False)
 
     copy_ms = eng.stream_copy(args.copy_mb << 20, reps=10)
     copy_gbs = 2 * (args.copy_mb << 20) / (copy_ms / 1e3) / 1e9
@@ -121,36 +178,46 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(eng, cfg, xyz, q0, dt, args.cpu_steps)
 
+    kern = "fp32 split step (stage 1 with Delta^4, stages 2-4 without)"
+    traffic = load_traffic(kern)
     out = {
         "metric": "RK4 point-updates/s @655362x60, FP32+closure vs FP64 plain; l2 err",
-        "value": v32,
+        "value": value["f32+closure"],
         "unit": "point-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": res["f32"][0] / args.steps,
+        "ms_per_step": res["f32+closure"][0] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (generated icosahedral nodes, analytic flow and tracers)",
-        "config": {"workload": f"mixed-precision RK4, icosahedral {N} nodes x {cfg.Nv} levels x {ntr} tracers, "
-                               f"deformational flow, n={cfg.n} deg 4, vertical n=7, Delta^4 hyperviscosity",
+        "data": "synthetic (generated icosahedral nodes, analytic flow and tracers, seed 2024)",
+        "config": {"workload": f"mixed-precision RK4 + learned closure, icosahedral {N} nodes x {cfg.Nv} levels x "
+                               f"{ntr} tracers, deformational flow, n={cfg.n} deg 4, vertical n=7, Delta^4 "
+                               "hyperviscosity",
                    "nodes": N, "levels": cfg.Nv, "tracers": ntr, "stencil": cfg.n, "dt_s": dt,
-                   "point_update": "one (node, level, tracer) advanced one RK4 step"},
-        "fp64": {"value": v64, "ms_per_step": res["f64"][0] / args.steps, "stage_us": res["f64"][1] * 1e3,
-                 "roofline_achieved_GBs": achieved64, "roofline_frac": achieved64 / HBM_PEAK_GBS},
-        "speedup_f32_vs_f64": v32 / v64,
-        "error": {"f32_vs_f64_rel_l2": err32, "f32_vs_f64_rel_linf": linf32, "after_steps": args.steps,
-                  "rel_l2_change_from_q0": drift},
+                   "point_update": "one (node, level) advanced one RK4 step (all 4 tracers)",
+                   "closure": closure_kind, "closure_every_steps": 8},
+        "fp64_plain": {"value": value["f64"], "ms_per_step": res["f64"][0] / args.steps,
+                       "stage_us": res["f64"][1] * 1e3, "roofline_achieved_GBs": achieved64,
+                       "roofline_frac": achieved64 / HBM_PEAK_GBS},
+        "f32_no_closure": {"value": value["f32"], "ms_per_step": res["f32"][0] / args.steps},
+        "speedup_vs_fp64_plain": value["f32+closure"] / value["f64"],
+        "tracer_point_updates_per_s": value["f32+closure"] * ntr,
+        "error": {"after_steps": args.steps, "vs_f64_leg": short, "period": period},
         "stream_copy_GBs": copy_gbs,
         "setup": {"knn_s": knn_s, "weights_s": w_s, "weight_solves_per_s": 2 * N / w_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage_kernel<float,64,*,4,true>", "bytes_per_launch": b32},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "stage2_kernel<float, 64, 4, STAGE, HV, 8>, average of the 4 launches of a step",
+                     "bytes_per_launch": b32,
+                     "note": "average FP32 stage launch of the f32 leg (HIP events); algorithmic bytes model "
+                             "bench.stage_bytes; traffic = calibrated FETCH_SIZE + WRITE_SIZE per launch"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    closure.close()
     eng.close()
 
 
@@ -177,7 +244,7 @@ def cpu_baseline(eng, cfg, xyz, q0, dt, steps):
     eng.run(F64, steps, dt)
     parity = float(np.linalg.norm(eng.state(F64) - q_cpu) / np.linalg.norm(q_cpu))
     log(f"full-scale parity, GPU FP64 vs oracle after {steps} steps: rel l2 {parity:.3e}")
-    pts = len(xyz) * cfg.Nv * max(1, cfg.ntracers)
+    pts = len(xyz) * cfg.Nv
     log(f"cpu baseline: {steps} steps in {s:.2f}s on {cores} cores")
     return {"value": pts * steps / s, "unit": "point-updates/s", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
             "kind": "port", "sample": f"{steps} RK4 steps of the full workload (FP64 oracle, OpenMP over nodes)",

@@ -169,6 +169,57 @@ int serbf_tile_info(serbf_engine *e, int *ntiles, int *umax);
  * copy kernel); *ms_per_copy is the average launch time.  Reference for the
  * attainable HBM bandwidth and for calibrating the HBM byte counters. */
 int serbf_stream_copy(serbf_engine *e, size_t bytes, int reps, double *ms_per_copy);
+/* Delta and Delta^2 weights L1, L2 double[N*n] (config mode, hv_order >= 3): solved together
+ * with the Delta^k weights from the same PHS r^K saddle system; used by the closure features. */
+int serbf_get_lap_weights(serbf_engine *e, double *L1, double *L2);
+
+/* ---- learned hyperviscosity closure ----
+ * A per-point multiplier s = softplus(MLP(x)) of the hyperviscosity coefficient,
+ * c = hv_coef * s, with the MLP 10 -> 64 -> 64 -> 1 (tanh hidden layers) evaluated in BF16
+ * on the matrix cores with FP32 accumulation.  s = 1 is the constant-coefficient method,
+ * which is also the fallback wherever s is not finite or exceeds s_max.
+ *
+ * Features of a point (node p, tracer, level v), with h the node's nearest-neighbour chord
+ * (unit sphere) and asinh compression a(y) = asinh(8 y):
+ *   x0 a(h |grad q|)   x1 a(h^2 Delta q)   x2 a(h^4 Delta^2 q)   x3 a(h^8 Delta^k q)
+ *   x4 h / hbar (hbar = sqrt(8 pi / (sqrt(3) N)))   x5 |u| dt / (R h)   x6 |u| / u_max
+ *   x7 (v + 1/2) / Nv   x8 q   x9 a(dh |dq/dz|)
+ * Network input: (x_i - shift_i) * scale_i.
+ * Parameters (float[SERBF_CLOSURE_PARAMS]): W1[64][10], b1[64], W2[64][64], b2[64], w3[64], b3. */
+#define SERBF_CLOSURE_FEATURES 10
+#define SERBF_CLOSURE_HIDDEN 64
+#define SERBF_CLOSURE_PARAMS 4929
+typedef struct serbf_closure serbf_closure;
+int serbf_closure_create(serbf_closure **out, int device);
+void serbf_closure_destroy(serbf_closure *c);
+/* params float[SERBF_CLOSURE_PARAMS]; norm float[32] = shift[16], scale[16] (first 10 used).
+ * Resets the Adam moments. */
+int serbf_closure_set_params(serbf_closure *c, const float *params, const float *norm);
+int serbf_closure_get_params(serbf_closure *c, float *params);
+/* Training set: n samples of features feat[n][16] (first 10 used, raw as above), model
+ * column A[n] and target R[n].  The loss of a sample is ((s - 1) A - R)^2, so s = 1 scores
+ * mean(R^2).  Samples are used in the given order (shuffle beforehand). */
+int serbf_closure_set_data(serbf_closure *c, long n, const float *feat, const float *A, const float *R);
+/* Mean loss over samples [first, first + count) and its gradient: grad float[SERBF_CLOSURE_PARAMS + 1]
+ * (the gradient, then the loss).  count must be a multiple of 16. */
+int serbf_closure_grad(serbf_closure *c, long first, long count, float *grad);
+/* steps Adam steps (beta 0.9, 0.999, eps 1e-8) over consecutive batches of `batch` samples
+ * (a multiple of 16) of the range [first, first + count), continuing from the batch after the
+ * last step taken and wrapping over the range; losses[steps] gets each batch's mean loss. */
+int serbf_closure_train(serbf_closure *c, long first, long count, int steps, int batch, double lr,
+                        float *losses);
+/* Network output o (before the softplus) for samples [first, first + count), count % 16 == 0. */
+int serbf_closure_eval(serbf_closure *c, long first, long count, float *o);
+/* Features of the FP64 state at time t for time step dt: feat[(p*ntr + tr)*Nv + v][16] and
+ * lap[...] = Delta^k q (unit sphere).  Needs config-mode weights with hv_order >= 3. */
+int serbf_closure_features(serbf_engine *e, double t, double dt, float *feat, double *lap);
+/* Use the closure in the engine's time loop: the coefficients are re-evaluated from the
+ * step-start state every `every` steps (1..8).  c = NULL returns to the constant coefficient. */
+int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_max);
+/* Evaluate the attached closure once on the state of the given precision at time t (step dt)
+ * and return the coefficients gamma double[N*ntr*Nv] (1/s). */
+int serbf_closure_gamma(serbf_engine *e, int precision, double t, double dt, double *gamma);
+
 /* Reset the FP32 state from the FP64 state. */
 int serbf_sync_f32_from_f64(serbf_engine *e);
 /* Relative norms of state(precision) against q_ref (host, N*Nv):

@@ -226,14 +226,15 @@ static double lap_k_phs_coef(int K, int k)
  * centre is the origin and scaled by the stencil radius s; PHS r^k + monomials
  * x^(p-i) y^i (p <= d).  Gradient weights are divided by s, Delta^k weights by s^2k. */
 static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, const double H[9],
-                            double *wx, double *wy, double *wl, int *info2)
+                            double *wx, double *wy, double *wl, double *wl1, double *wl2,
+                            int *info2)
 {
     const int d = cfg->poly_deg, M = (d + 1) * (d + 2) / 2, m = n + M;
     double *xs = malloc(sizeof(double) * 3 * n);
     double *A = malloc(sizeof(double) * (size_t)m * m);
     double *B = malloc(sizeof(double) * (size_t)m * 2);
     double *A2 = malloc(sizeof(double) * (size_t)m * m);
-    double *B2 = malloc(sizeof(double) * (size_t)m);
+    double *B2 = malloc(sizeof(double) * (size_t)m * 3);
     int *ipiv = malloc(sizeof(int) * m);
     double c0[3];
     for (int r = 0; r < 3; r++)
@@ -264,7 +265,10 @@ static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, con
         double r = sqrt(dx * dx + dy * dy + dz * dz);
         B[2 * i] = k * dx * pow(r, k - 2);
         B[2 * i + 1] = k * dy * pow(r, k - 2);
-        B2[i] = kk > 0 ? cL * pow(r, K - 2 * kk) : 0.0;
+        /* right-hand sides Delta^kk, Delta, Delta^2 of r^K at the centre */
+        B2[3 * i] = kk > 0 ? cL * pow(r, K - 2 * kk) : 0.0;
+        B2[3 * i + 1] = lap_k_phs_coef(K, 1) * pow(r, K - 2);
+        B2[3 * i + 2] = lap_k_phs_coef(K, 2) * pow(r, K - 4);
     }
     int off = n;
     for (int p = 0; p <= d; p++) {
@@ -278,7 +282,11 @@ static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, con
             }
             B[2 * (off + i)] = (p == 1 && i == 0) ? 1.0 : 0.0;
             B[2 * (off + i) + 1] = (p == 1 && i == 1) ? 1.0 : 0.0;
-            B2[off + i] = 0.0;
+            /* Delta, Delta^2 of x^(p-i) y^i at the origin: 2 for x^2, y^2; 24, 8, 24 for
+             * x^4, x^2 y^2, y^4; Delta^kk (kk >= 3) of degree < 2kk vanishes there */
+            B2[3 * (off + i)] = 0.0;
+            B2[3 * (off + i) + 1] = (p == 2 && (i == 0 || i == 2)) ? 2.0 : 0.0;
+            B2[3 * (off + i) + 2] = p == 4 ? ((i == 0 || i == 4) ? 24.0 : (i == 2 ? 8.0 : 0.0)) : 0.0;
         }
         off += p + 1;
     }
@@ -290,12 +298,14 @@ static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, con
     int info1 = orc_dgesv(m, 2, A, ipiv, B);
     *info2 = 0;
     if (kk > 0)
-        *info2 = orc_dgesv(m, 1, A2, ipiv, B2);
+        *info2 = orc_dgesv(m, 3, A2, ipiv, B2);
     double s2k = pow(s, 2 * kk);
     for (int j = 0; j < n; j++) {
         wx[j] = B[2 * j] / s;
         wy[j] = B[2 * j + 1] / s;
-        wl[j] = kk > 0 ? B2[j] / s2k : 0.0;
+        wl[j] = kk > 0 ? B2[3 * j] / s2k : 0.0;
+        if (wl1) wl1[j] = kk > 0 ? B2[3 * j + 1] / (s * s) : 0.0;
+        if (wl2) wl2[j] = kk > 0 ? B2[3 * j + 2] / (s * s * s * s) : 0.0;
     }
     free(xs); free(A); free(B); free(A2); free(B2); free(ipiv);
     return info1;
@@ -304,6 +314,13 @@ static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, con
 int orc_weights(const serbf_config *cfg, int N, const double *x, const double *y,
                 const double *z, const int *idx, double *Dx, double *Dy, double *L,
                 int *status)
+{
+    return orc_weights_lap(cfg, N, x, y, z, idx, Dx, Dy, L, NULL, NULL, status);
+}
+
+int orc_weights_lap(const serbf_config *cfg, int N, const double *x, const double *y,
+                    const double *z, const int *idx, double *Dx, double *Dy, double *L,
+                    double *L1, double *L2, int *status)
 {
     const int n = cfg->n;
     int bad = 0;
@@ -326,7 +343,8 @@ int orc_weights(const serbf_config *cfg, int N, const double *x, const double *y
         } else {
             orc_householder(x[i], y[i], z[i], 1, H);
             info1 = cfg_node_weights(cfg, n, X, H, Dx + (size_t)i * n, Dy + (size_t)i * n,
-                                     L + (size_t)i * n, &info2);
+                                     L + (size_t)i * n, L1 ? L1 + (size_t)i * n : NULL,
+                                     L2 ? L2 + (size_t)i * n : NULL, &info2);
         }
         status[i] = info1 ? 1 : (info2 ? 2 : 0);
         bad += status[i] != 0;

@@ -1,0 +1,858 @@
+// Learned hyperviscosity closure (gfx950): per-point features, the 10 -> 64 -> 64 -> 1 MLP in
+// BF16 on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulation), its backward pass,
+// and Adam.
+//
+// MFMA operand layout (16x16x16, lane l, c = l % 16, g = l / 16):
+//   A (M x K): A[c][4g + j]      B (K x N): B[4g + j][c]      D (M x N): D[4g + i][c]
+// Layer 1 is computed as Z1 = W1 X (M = hidden unit, N = sample), so its output tile is
+// already the A operand of layer 2, Z2^T = H1^T W2^T (M = sample, N = unit).  The backward
+// pass needs three transposes (H1, dZ2, dZ1), staged through the wavefront's LDS.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+#include "stage_common.h"
+
+struct serbf_closure {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float *params = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    unsigned short *w1b = nullptr, *w2b = nullptr, *w2tb = nullptr;
+    float *norm = nullptr;
+    int step = 0;
+    bool have_params = false;
+    long n = 0, cap = 0;
+    float *feat = nullptr, *A = nullptr, *R = nullptr;
+    float *partials = nullptr, *grad = nullptr, *losses = nullptr;
+    int loss_cap = 0;
+    float *out = nullptr;
+    long out_cap = 0;
+};
+
+namespace serbf {
+namespace {
+
+constexpr int kF = SERBF_CLOSURE_FEATURES;
+constexpr int kH = SERBF_CLOSURE_HIDDEN;
+constexpr int kP = SERBF_CLOSURE_PARAMS;
+constexpr int kPS = 4936;   // partial-gradient record: parameters, loss, padding
+constexpr int oB1 = kF * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH, oLoss = kP;
+constexpr int kMaxWaves = 1024;
+constexpr float kKappa = 8.0f;
+static_assert(oB3 + 1 == kP, "parameter layout");
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// round to nearest even
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ v4s pack4(float a, float b, float c, float d)
+{
+    v4s r;
+    r[0] = (short)f2bf(a);
+    r[1] = (short)f2bf(b);
+    r[2] = (short)f2bf(c);
+    r[3] = (short)f2bf(d);
+    return r;
+}
+__device__ __forceinline__ v4s ld4(const unsigned short *p) { return *(const v4s *)p; }
+__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float softplus_f(float o) { return o > 20.f ? o : log1pf(expf(o)); }
+__device__ __forceinline__ float sigmoid_f(float o) { return 1.f / (1.f + expf(-o)); }
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+struct NetView {
+    const float *params;
+    const unsigned short *w1b;    // W1 [64][16] (features 10..15 zero)
+    const unsigned short *w2b;    // W2 [64][64], row = output unit
+    const unsigned short *w2tb;   // W2^T [64][64]
+    const float *norm;            // shift[16], scale[16]
+};
+
+NetView view(const serbf_closure *c) { return NetView{c->params, c->w1b, c->w2b, c->w2tb, c->norm}; }
+
+// forward-pass operands held in registers by every lane
+struct FwdRegs {
+    v4s w1[4];
+    v4s w2[4][4];
+    float b1[4][4];
+    float b2[4], w3[4];
+    float b3;
+};
+
+__device__ __forceinline__ void load_fwd(const NetView &nv, int lane, FwdRegs &r)
+{
+    const int c = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; t++) r.w1[t] = ld4(nv.w1b + (16 * t + c) * 16 + 4 * g);
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) r.w2[kk][t] = ld4(nv.w2b + (16 * t + c) * kH + 16 * kk + 4 * g);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) r.b1[t][i] = nv.params[oB1 + 16 * t + 4 * g + i];
+        r.b2[t] = nv.params[oB2 + 16 * t + c];
+        r.w3[t] = nv.params[oW3 + 16 * t + c];
+    }
+    r.b3 = nv.params[oB3];
+}
+
+// One tile of 16 samples.  xb = X[feature 4g + j][sample c].  Out: h1[t][i] = H1[unit 16t + 4g + i]
+// [sample c], h2[t][i] = H2[sample 4g + i][unit 16t + c], o[i] = output of sample 4g + i.
+__device__ __forceinline__ void fwd_tile(const FwdRegs &r, v4s xb, float h1[4][4], float h2[4][4], float o[4])
+{
+    const v4f zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const v4f z = mfma(r.w1[t], xb, zero);
+#pragma unroll
+        for (int i = 0; i < 4; i++) h1[t][i] = tanhf(z[i] + r.b1[t][i]);
+    }
+    v4f z2[4] = {zero, zero, zero, zero};
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+        const v4s a = pack4(h1[kk][0], h1[kk][1], h1[kk][2], h1[kk][3]);
+#pragma unroll
+        for (int t = 0; t < 4; t++) z2[t] = mfma(a, r.w2[kk][t], z2[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) h2[t][i] = tanhf(z2[t][i] + r.b2[t]);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        float s = 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; t++) s = fmaf(r.w3[t], h2[t][i], s);
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) s += __shfl_xor(s, m, 64);
+        o[i] = s + r.b3;
+    }
+}
+
+__device__ __forceinline__ float pick4(const float o[4], int i)
+{
+    return i == 0 ? o[0] : (i == 1 ? o[1] : (i == 2 ? o[2] : o[3]));
+}
+
+// ---------------------------------------------------------------------------
+// features
+
+template <typename T>
+struct FeatArgs {
+    StageArgs<T> s;          // state (qin), stencil, weights, basis, levels, vertical table, times
+    const T *WL1, *WL2;
+    const float *hloc;
+    int ntr;
+    float hbar, umax, R, dh;
+};
+
+template <typename T, int S>
+__device__ void point_features(const FeatArgs<T> &a, int p, int tr, int v, float f[kF], T &l4out)
+{
+    const StageArgs<T> &s = a.s;
+    const int NT = a.ntr;
+    const int vv = v < s.Nv ? v : s.Nv - 1;
+    T gx = T(0), gy = T(0), l1 = T(0), l2 = T(0), l4 = T(0);
+    const size_t row = (size_t)p * s.n;
+    const T *qcol = s.qin + (size_t)tr * S + vv;
+    for (int k = 0; k < s.n; k++) {
+        const T q = qcol[(size_t)s.idx[row + k] * NT * S];
+        gx = fma(s.Wx[row + k], q, gx);
+        gy = fma(s.Wy[row + k], q, gy);
+        l1 = fma(a.WL1[row + k], q, l1);
+        l2 = fma(a.WL2[row + k], q, l2);
+        l4 = fma(s.WL[row + k], q, l4);
+    }
+    const T *own = s.qin + ((size_t)p * NT + tr) * S;
+    T dz = T(0);
+    const int st = s.vstart[vv];
+    for (int j = 0; j < s.nv; j++) {
+        const int k = st + j;
+        if (k >= 0 && k < s.Nv) dz = fma(s.vW[vv * kMaxNv + j], own[k], dz);
+    }
+    const T *ts = s.tscal + ((size_t)(*s.base + s.step_off) * 4) * kTimeScalars;
+    T ux, uy, uz;
+    level_velocity<T, S>(s, s.basis + (size_t)p * kBasis, ts, p, vv, ux, uy, uz);
+    const T speed = sqrt(ux * ux + uy * uy + uz * uz);
+    const T h = (T)a.hloc[p];
+    const T h2 = h * h, h4 = h2 * h2, h8 = h4 * h4;
+    const T K = (T)kKappa;
+    f[0] = (float)asinh(K * h * sqrt(gx * gx + gy * gy));
+    f[1] = (float)asinh(K * h2 * l1);
+    f[2] = (float)asinh(K * h4 * l2);
+    f[3] = (float)asinh(K * h8 * l4);
+    f[4] = (float)(h / (T)a.hbar);
+    f[5] = (float)(speed * s.dt / ((T)a.R * h));
+    f[6] = (float)(speed / (T)a.umax);
+    f[7] = (float)((T(vv) + T(0.5)) / T(s.Nv));
+    f[8] = (float)own[vv];
+    f[9] = (float)asinh(K * (T)a.dh * fabs(dz));
+    l4out = l4;
+}
+
+// feat[(col * Nv + v) * 16 + k], lap[col * Nv + v]; a wavefront holds 64 / S columns
+template <typename T, int S>
+__global__ void __launch_bounds__(256) features_kernel(FeatArgs<T> a, float *feat, T *lap)
+{
+    constexpr int CPW = 64 / S;
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long col = wave * CPW + lane / S;
+    const int v = lane % S;
+    const long ncol = (long)a.s.N * a.ntr;
+    if (col >= ncol || v >= a.s.Nv) return;
+    const int p = (int)(col / a.ntr), tr = (int)(col % a.ntr);
+    float f[kF];
+    T l4;
+    point_features<T, S>(a, p, tr, v, f, l4);
+    const size_t smp = (size_t)col * a.s.Nv + v;
+#pragma unroll
+    for (int k = 0; k < 16; k++) feat[smp * 16 + k] = k < kF ? f[k] : 0.f;
+    lap[smp] = l4;
+}
+
+// closure coefficients: gamma[(p*ntr + tr)*S + v] = hv_coef * s, s = softplus(MLP(x)), with
+// s = 1 where s is not finite or above s_max; levels >= Nv get 0
+template <typename T, int S>
+__global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, float hv_coef, float s_max)
+{
+    constexpr int CPW = 64 / S;
+    __shared__ alignas(16) unsigned short xs[4][64 * 16];
+    __shared__ float os[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const long wave = (long)blockIdx.x * 4 + w;
+    const long ncol = (long)a.s.N * a.ntr;
+    if (wave * CPW >= ncol) return;   // whole wavefronts only
+    long col = wave * CPW + lane / S;
+    const bool ok = col < ncol;
+    if (!ok) col = ncol - 1;
+    const int v = lane % S;
+    const int p = (int)(col / a.ntr), tr = (int)(col % a.ntr);
+    float f[kF];
+    T l4;
+    point_features<T, S>(a, p, tr, v, f, l4);
+    unsigned short *xrow = &xs[w][lane * 16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const float x = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
+        xrow[k] = f2bf(x);
+    }
+    wave_sync();
+    FwdRegs r;
+    load_fwd(nv, lane, r);
+#pragma unroll 1
+    for (int st = 0; st < 4; st++) {
+        const v4s xb = ld4(&xs[w][(16 * st + c) * 16 + 4 * g]);
+        float h1[4][4], h2[4][4], o[4];
+        fwd_tile(r, xb, h1, h2, o);
+        if (c < 4) os[w][16 * st + 4 * g + c] = pick4(o, c);
+    }
+    wave_sync();
+    float sv = softplus_f(os[w][lane]);
+    if (!(sv >= 0.f && sv <= s_max)) sv = 1.f;
+    if (ok) gamma[((size_t)p * a.ntr + tr) * S + v] = v < a.s.Nv ? (T)(hv_coef * sv) : T(0);
+}
+
+// ---------------------------------------------------------------------------
+// training
+
+// one wavefront per strided set of 16-sample tiles; writes its gradient record to partials
+__global__ void __launch_bounds__(256) train_kernel(NetView nv, const float *feat, const float *A, const float *R,
+                                                    long first, int ntiles, float inv_count, float *partials)
+{
+    __shared__ alignas(16) unsigned short w2t[kH * kH];
+    __shared__ alignas(16) unsigned short h1s[4][kH * 16];
+    __shared__ alignas(16) unsigned short dz2s[4][16 * kH];
+    __shared__ alignas(16) unsigned short dz1s[4][kH * 16];
+    __shared__ alignas(16) unsigned short xs[4][16 * 16];
+    for (int k = threadIdx.x; k < kH * kH; k += blockDim.x) w2t[k] = nv.w2tb[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int wave = blockIdx.x * 4 + w, nwaves = gridDim.x * 4;
+    FwdRegs r;
+    load_fwd(nv, lane, r);
+    float nsh[4], nsc[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        nsh[j] = nv.norm[4 * g + j];
+        nsc[j] = (4 * g + j) < kF ? nv.norm[16 + 4 * g + j] : 0.f;
+    }
+    const v4f zero = {0.f, 0.f, 0.f, 0.f};
+    v4f dw1[4] = {zero, zero, zero, zero};
+    v4f dw2[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) dw2[a][b] = zero;
+    float db1[4][4] = {}, db2[4] = {}, dw3[4] = {};
+    float db3 = 0.f, loss = 0.f;
+    for (int tile = wave; tile < ntiles; tile += nwaves) {
+        const long s0 = first + (long)tile * 16;
+        const float4 fv = *(const float4 *)(feat + (s0 + c) * 16 + 4 * g);
+        const float x0 = (fv.x - nsh[0]) * nsc[0], x1 = (fv.y - nsh[1]) * nsc[1];
+        const float x2 = (fv.z - nsh[2]) * nsc[2], x3 = (fv.w - nsh[3]) * nsc[3];
+        const v4s xb = pack4(x0, x1, x2, x3);
+        *(v4s *)&xs[w][c * 16 + 4 * g] = xb;   // X[sample c][feature 4g + j]
+        float h1[4][4], h2[4][4], o[4];
+        fwd_tile(r, xb, h1, h2, o);
+        float go[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const long si = s0 + 4 * g + i;
+            const float av = A[si], rv = R[si];
+            const float e = (softplus_f(o[i]) - 1.f) * av - rv;
+            if (c == 0) loss = fmaf(e, e, loss);
+            go[i] = 2.f * e * av * sigmoid_f(o[i]) * inv_count;
+        }
+        if (c == 0) db3 += (go[0] + go[1]) + (go[2] + go[3]);
+        float dz2[4][4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            float acc = 0.f, bsum = 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                acc = fmaf(go[i], h2[t][i], acc);
+                dz2[t][i] = go[i] * r.w3[t] * (1.f - h2[t][i] * h2[t][i]);
+                bsum += dz2[t][i];
+            }
+            dw3[t] += acc;
+            db2[t] += bsum;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                h1s[w][(16 * t + 4 * g + i) * 16 + c] = f2bf(h1[t][i]);      // H1[unit][sample]
+                dz2s[w][(4 * g + i) * kH + 16 * t + c] = f2bf(dz2[t][i]);    // dZ2[sample][unit]
+            }
+        wave_sync();
+        // dW2[n][h] += sum_s dZ2[s][n] H1[s][h]   (M = n, N = h, K = s)
+#pragma unroll
+        for (int tm = 0; tm < 4; tm++) {
+            const v4s a = pack4(dz2[tm][0], dz2[tm][1], dz2[tm][2], dz2[tm][3]);
+#pragma unroll
+            for (int tn = 0; tn < 4; tn++) dw2[tm][tn] = mfma(a, ld4(&h1s[w][(16 * tn + c) * 16 + 4 * g]), dw2[tm][tn]);
+        }
+        // dH1^T[h][s] = sum_n W2^T[h][n] dZ2^T[n][s]   (M = h, N = s, K = n)
+        float dz1[4][4];
+#pragma unroll
+        for (int tm = 0; tm < 4; tm++) {
+            v4f acc = zero;
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++)
+                acc = mfma(ld4(&w2t[(16 * tm + c) * kH + 16 * kk + 4 * g]), ld4(&dz2s[w][c * kH + 16 * kk + 4 * g]),
+                           acc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                dz1[tm][i] = acc[i] * (1.f - h1[tm][i] * h1[tm][i]);
+                db1[tm][i] += dz1[tm][i];
+                dz1s[w][(16 * tm + 4 * g + i) * 16 + c] = f2bf(dz1[tm][i]);   // dZ1[unit][sample]
+            }
+        }
+        wave_sync();
+        // dW1[h][f] += sum_s dZ1[s][h] X[s][f]   (M = h, N = f, K = s)
+        v4s bx;
+#pragma unroll
+        for (int j = 0; j < 4; j++) bx[j] = (short)xs[w][(4 * g + j) * 16 + c];
+#pragma unroll
+        for (int tm = 0; tm < 4; tm++) dw1[tm] = mfma(ld4(&dz1s[w][(16 * tm + c) * 16 + 4 * g]), bx, dw1[tm]);
+        wave_sync();
+    }
+    float *P = partials + (size_t)wave * kPS;
+#pragma unroll
+    for (int tm = 0; tm < 4; tm++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int h = 16 * tm + 4 * g + i;
+            if (c < kF) P[h * kF + c] = dw1[tm][i];
+#pragma unroll
+            for (int tn = 0; tn < 4; tn++) P[oW2 + h * kH + 16 * tn + c] = dw2[tm][tn][i];
+            float b = db1[tm][i];
+#pragma unroll
+            for (int m = 1; m < 16; m <<= 1) b += __shfl_xor(b, m, 64);
+            if (c == 0) P[oB1 + h] = b;
+        }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        float a = db2[t], b = dw3[t];
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 32, 64);
+        b += __shfl_xor(b, 16, 64);
+        b += __shfl_xor(b, 32, 64);
+        if (g == 0) {
+            P[oB2 + 16 * t + c] = a;
+            P[oW3 + 16 * t + c] = b;
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        db3 += __shfl_xor(db3, m, 64);
+        loss += __shfl_xor(loss, m, 64);
+    }
+    if (lane == 0) {
+        P[oB3] = db3;
+        P[oLoss] = loss;
+    }
+}
+
+__device__ __forceinline__ void store_bf16_copies(int k, float p, unsigned short *w1b, unsigned short *w2b,
+                                                  unsigned short *w2tb)
+{
+    if (k < oB1) {
+        w1b[(k / kF) * 16 + k % kF] = f2bf(p);
+    } else if (k >= oW2 && k < oB2) {
+        const int q = k - oW2, n = q / kH, h = q % kH;
+        w2b[n * kH + h] = f2bf(p);
+        w2tb[h * kH + n] = f2bf(p);
+    }
+}
+
+// sum of the wavefront records; optional Adam step on the parameters and their BF16 copies
+__global__ void reduce_adam_kernel(float *params, float *m, float *v, unsigned short *w1b, unsigned short *w2b,
+                                   unsigned short *w2tb, const float *partials, int nwaves, float *grad,
+                                   float *losses, int loss_idx, float loss_scale, int update, float lr, float b1,
+                                   float b2, float eps, float bc1, float bc2)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > oLoss) return;
+    float s = 0.f;
+    for (int w = 0; w < nwaves; w++) s += partials[(size_t)w * kPS + k];
+    if (k == oLoss) {
+        s *= loss_scale;
+        grad[k] = s;
+        if (losses) losses[loss_idx] = s;
+        return;
+    }
+    grad[k] = s;
+    if (!update) return;
+    const float mk = b1 * m[k] + (1.f - b1) * s;
+    const float vk = b2 * v[k] + (1.f - b2) * s * s;
+    m[k] = mk;
+    v[k] = vk;
+    const float p = params[k] - lr * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+    params[k] = p;
+    store_bf16_copies(k, p, w1b, w2b, w2tb);
+}
+
+__global__ void pack_kernel(const float *params, unsigned short *w1b, unsigned short *w2b, unsigned short *w2tb)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < kH * 16 && k % 16 >= kF) w1b[k] = 0;
+    if (k < kP) store_bf16_copies(k, params[k], w1b, w2b, w2tb);
+}
+
+__global__ void __launch_bounds__(256) eval_kernel(NetView nv, const float *feat, long first, int ntiles, float *out)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int wave = blockIdx.x * 4 + w, nwaves = gridDim.x * 4;
+    FwdRegs r;
+    load_fwd(nv, lane, r);
+    float nsh[4], nsc[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        nsh[j] = nv.norm[4 * g + j];
+        nsc[j] = (4 * g + j) < kF ? nv.norm[16 + 4 * g + j] : 0.f;
+    }
+    for (int tile = wave; tile < ntiles; tile += nwaves) {
+        const long s0 = first + (long)tile * 16;
+        const float4 fv = *(const float4 *)(feat + (s0 + c) * 16 + 4 * g);
+        const v4s xb = pack4((fv.x - nsh[0]) * nsc[0], (fv.y - nsh[1]) * nsc[1], (fv.z - nsh[2]) * nsc[2],
+                             (fv.w - nsh[3]) * nsc[3]);
+        float h1[4][4], h2[4][4], o[4];
+        fwd_tile(r, xb, h1, h2, o);
+        if (c < 4) out[s0 - first + 4 * g + c] = pick4(o, c);
+    }
+}
+
+// nearest-neighbour chord of every node (stencil entry 1 is the nearest after the node itself)
+__global__ void hloc_kernel(int N, int n, const double *x, const double *y, const double *z, const int *idx,
+                            float *h)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const int q = idx[(size_t)p * n + (n > 1 ? 1 : 0)];
+    const double dx = x[q] - x[p], dy = y[q] - y[p], dz = z[q] - z[p];
+    h[p] = (float)sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+int launch_grid_waves(int ntiles)
+{
+    int waves = ntiles < kMaxWaves ? ntiles : kMaxWaves;
+    if (waves < 4) waves = 4;
+    return (waves + 3) / 4;   // blocks of 4 wavefronts
+}
+
+int run_grad(serbf_closure *c, long first, long count, bool update, float lr, int loss_idx)
+{
+    SERBF_CHECK(c->have_params, SERBF_ERR_STATE, "closure: parameters not set");
+    SERBF_CHECK(count > 0 && count % 16 == 0 && first >= 0 && first + count <= c->n, SERBF_ERR_ARG,
+                "closure: sample range must lie in the set and be a multiple of 16");
+    const int ntiles = (int)(count / 16);
+    const int blocks = launch_grid_waves(ntiles);
+    hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(256), 0, c->stream, view(c), c->feat, c->A, c->R, first,
+                       ntiles, 1.f / (float)count, c->partials);
+    SERBF_HIP(hipGetLastError());
+    float bc1 = 1.f, bc2 = 1.f;
+    if (update) {
+        c->step++;
+        bc1 = 1.f - powf(0.9f, (float)c->step);
+        bc2 = 1.f - powf(0.999f, (float)c->step);
+    }
+    hipLaunchKernelGGL(reduce_adam_kernel, dim3((kPS + 255) / 256), dim3(256), 0, c->stream, c->params, c->adam_m,
+                       c->adam_v, c->w1b, c->w2b, c->w2tb, c->partials, blocks * 4, c->grad,
+                       loss_idx >= 0 ? c->losses : nullptr, loss_idx, 1.f / (float)count, update ? 1 : 0, lr, 0.9f,
+                       0.999f, 1e-8f, bc1, bc2);
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
+template <typename T>
+FeatArgs<T> feat_args(serbf_engine *e, StateBuffers<T> &sb, const T *q, double dt, const T *tscal, const int *base,
+                      int step_off)
+{
+    FeatArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    StageArgs<T> &s = a.s;
+    s.qin = q;
+    s.idx = e->d_idx;
+    s.Wx = sb.Wx; s.Wy = sb.Wy; s.WL = sb.WL;
+    s.basis = sb.basis;
+    s.lev = sb.lev;
+    s.vstart = e->d_vstart;
+    s.vW = sb.vW;
+    s.tscal = tscal;
+    s.base = base;
+    s.step_off = step_off;
+    s.N = e->N; s.n = e->n; s.Nv = e->cfg.Nv; s.nv = e->nv;
+    s.flow = e->cfg.flow;
+    s.ref_mode = e->cfg.mode == SERBF_MODE_REFERENCE;
+    s.dt = (T)dt;
+    a.WL1 = sb.WL1;
+    a.WL2 = sb.WL2;
+    a.hloc = e->d_hloc;
+    a.ntr = e->ntr;
+    a.hbar = (float)std::sqrt(8.0 * SERBF_PI / (std::sqrt(3.0) * e->N));
+    a.umax = (float)(e->cfg.flow == SERBF_FLOW_DEFORM ? e->cfg.u0 + e->cfg.k_def : e->cfg.u0);
+    if (!(a.umax > 0.f)) a.umax = 1.f;
+    a.R = (float)e->cfg.R;
+    a.dh = (float)(e->cfg.htop / e->cfg.Nv);
+    return a;
+}
+
+// recomputed on every call (the stencils may have changed); allocated once per node set
+int ensure_hloc(serbf_engine *e)
+{
+    if (!e->d_hloc) SERBF_HIP(hipMalloc((void **)&e->d_hloc, (size_t)e->N * sizeof(float)));
+    hipLaunchKernelGGL(hloc_kernel, dim3((e->N + 255) / 256), dim3(256), 0, e->stream, e->N, e->n, e->d_x, e->d_y,
+                       e->d_z, e->d_idx, e->d_hloc);
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
+template <typename T>
+int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma)
+{
+    const long ncol = (long)e->N * e->ntr;
+    const long waves = e->S == 16 ? (ncol + 3) / 4 : ncol;
+    const dim3 grid((unsigned)((waves + 3) / 4));
+    const NetView nv = view(e->closure);
+    const float hv = (float)e->cfg.hv_coef, smax = (float)e->closure_smax;
+    if (e->S == 16) hipLaunchKernelGGL((apply_kernel<T, 16>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax);
+    else hipLaunchKernelGGL((apply_kernel<T, 64>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax);
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
+int alloc_gamma(serbf_engine *e)
+{
+    const size_t P = (size_t)e->N * e->ntr * e->S;
+    if (!e->s64.gamma) {
+        SERBF_HIP(hipMalloc((void **)&e->s64.gamma, P * sizeof(double)));
+        SERBF_HIP(hipMemsetAsync(e->s64.gamma, 0, P * sizeof(double), e->stream));
+    }
+    if (!e->s32.gamma) {
+        SERBF_HIP(hipMalloc((void **)&e->s32.gamma, P * sizeof(float)));
+        SERBF_HIP(hipMemsetAsync(e->s32.gamma, 0, P * sizeof(float), e->stream));
+    }
+    return SERBF_OK;
+}
+
+}  // namespace
+
+template <typename T>
+int closure_enqueue(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off)
+{
+    const FeatArgs<T> a = feat_args<T>(e, sb, sb.qT, dt, sb.tscal, e->d_base, step_off);
+    return launch_apply<T>(e, a, sb.gamma);
+}
+template int closure_enqueue<float>(serbf_engine *, StateBuffers<float> &, double, int);
+template int closure_enqueue<double>(serbf_engine *, StateBuffers<double> &, double, int);
+
+}  // namespace serbf
+
+using namespace serbf;
+
+extern "C" {
+
+int serbf_closure_create(serbf_closure **out, int device)
+{
+    SERBF_CHECK(out, SERBF_ERR_ARG, "closure_create: null");
+    SERBF_HIP(hipSetDevice(device));
+    serbf_closure *c = new serbf_closure();
+    c->device = device;
+    SERBF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    SERBF_HIP(hipMalloc((void **)&c->params, kP * sizeof(float)));
+    SERBF_HIP(hipMalloc((void **)&c->adam_m, kP * sizeof(float)));
+    SERBF_HIP(hipMalloc((void **)&c->adam_v, kP * sizeof(float)));
+    SERBF_HIP(hipMalloc((void **)&c->w1b, kH * 16 * sizeof(unsigned short)));
+    SERBF_HIP(hipMalloc((void **)&c->w2b, kH * kH * sizeof(unsigned short)));
+    SERBF_HIP(hipMalloc((void **)&c->w2tb, kH * kH * sizeof(unsigned short)));
+    SERBF_HIP(hipMalloc((void **)&c->norm, 32 * sizeof(float)));
+    SERBF_HIP(hipMalloc((void **)&c->partials, (size_t)(kMaxWaves + 4) * kPS * sizeof(float)));
+    SERBF_HIP(hipMalloc((void **)&c->grad, kPS * sizeof(float)));
+    *out = c;
+    return SERBF_OK;
+}
+
+void serbf_closure_destroy(serbf_closure *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (float *p : {c->params, c->adam_m, c->adam_v, c->norm, c->feat, c->A, c->R, c->partials, c->grad, c->losses,
+                     c->out})
+        if (p) (void)hipFree(p);
+    for (unsigned short *p : {c->w1b, c->w2b, c->w2tb})
+        if (p) (void)hipFree(p);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int serbf_closure_set_params(serbf_closure *c, const float *params, const float *norm)
+{
+    SERBF_CHECK(c && params && norm, SERBF_ERR_ARG, "closure_set_params: null");
+    SERBF_HIP(hipSetDevice(c->device));
+    SERBF_HIP(hipMemcpyAsync(c->params, params, kP * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SERBF_HIP(hipMemcpyAsync(c->norm, norm, 32 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SERBF_HIP(hipMemsetAsync(c->adam_m, 0, kP * sizeof(float), c->stream));
+    SERBF_HIP(hipMemsetAsync(c->adam_v, 0, kP * sizeof(float), c->stream));
+    hipLaunchKernelGGL(pack_kernel, dim3((kP + 255) / 256), dim3(256), 0, c->stream, c->params, c->w1b, c->w2b,
+                       c->w2tb);
+    SERBF_HIP(hipGetLastError());
+    SERBF_HIP(hipStreamSynchronize(c->stream));
+    c->step = 0;
+    c->have_params = true;
+    return SERBF_OK;
+}
+
+int serbf_closure_get_params(serbf_closure *c, float *params)
+{
+    SERBF_CHECK(c && params && c->have_params, SERBF_ERR_STATE, "closure_get_params: no parameters");
+    SERBF_HIP(hipSetDevice(c->device));
+    SERBF_HIP(hipMemcpyAsync(params, c->params, kP * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SERBF_HIP(hipStreamSynchronize(c->stream));
+    return SERBF_OK;
+}
+
+int serbf_closure_set_data(serbf_closure *c, long n, const float *feat, const float *A, const float *R)
+{
+    SERBF_CHECK(c && n > 0 && feat && A && R, SERBF_ERR_ARG, "closure_set_data: bad arguments");
+    SERBF_HIP(hipSetDevice(c->device));
+    if (n > c->cap) {
+        for (float **p : {&c->feat, &c->A, &c->R})
+            if (*p) { (void)hipFree(*p); *p = nullptr; }
+        SERBF_HIP(hipMalloc((void **)&c->feat, (size_t)n * 16 * sizeof(float)));
+        SERBF_HIP(hipMalloc((void **)&c->A, (size_t)n * sizeof(float)));
+        SERBF_HIP(hipMalloc((void **)&c->R, (size_t)n * sizeof(float)));
+        c->cap = n;
+    }
+    SERBF_HIP(hipMemcpyAsync(c->feat, feat, (size_t)n * 16 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SERBF_HIP(hipMemcpyAsync(c->A, A, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SERBF_HIP(hipMemcpyAsync(c->R, R, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SERBF_HIP(hipStreamSynchronize(c->stream));
+    c->n = n;
+    return SERBF_OK;
+}
+
+int serbf_closure_grad(serbf_closure *c, long first, long count, float *grad)
+{
+    SERBF_CHECK(c && grad, SERBF_ERR_ARG, "closure_grad: null");
+    SERBF_HIP(hipSetDevice(c->device));
+    int rc = run_grad(c, first, count, false, 0.f, -1);
+    if (rc) return rc;
+    SERBF_HIP(hipMemcpyAsync(grad, c->grad, (kP + 1) * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SERBF_HIP(hipStreamSynchronize(c->stream));
+    return SERBF_OK;
+}
+
+int serbf_closure_train(serbf_closure *c, long first, long count, int steps, int batch, double lr, float *losses)
+{
+    SERBF_CHECK(c && steps >= 0 && batch > 0 && batch % 16 == 0, SERBF_ERR_ARG, "closure_train: bad arguments");
+    SERBF_CHECK(first >= 0 && count >= batch && first + count <= c->n, SERBF_ERR_ARG,
+                "closure_train: the sample range must lie in the set and hold at least one batch");
+    SERBF_HIP(hipSetDevice(c->device));
+    if (steps > c->loss_cap) {
+        if (c->losses) (void)hipFree(c->losses);
+        SERBF_HIP(hipMalloc((void **)&c->losses, (size_t)steps * sizeof(float)));
+        c->loss_cap = steps;
+    }
+    const long nb = count / batch;
+    for (int s = 0; s < steps; s++) {
+        const long b0 = first + (long)((c->step) % nb) * batch;
+        int rc = run_grad(c, b0, batch, true, (float)lr, s);
+        if (rc) return rc;
+    }
+    if (losses && steps > 0)
+        SERBF_HIP(hipMemcpyAsync(losses, c->losses, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SERBF_HIP(hipStreamSynchronize(c->stream));
+    return SERBF_OK;
+}
+
+int serbf_closure_eval(serbf_closure *c, long first, long count, float *o)
+{
+    SERBF_CHECK(c && o && c->have_params, SERBF_ERR_STATE, "closure_eval: no parameters");
+    SERBF_CHECK(count > 0 && count % 16 == 0 && first >= 0 && first + count <= c->n, SERBF_ERR_ARG,
+                "closure_eval: sample range must lie in the set and be a multiple of 16");
+    SERBF_HIP(hipSetDevice(c->device));
+    if (count > c->out_cap) {
+        if (c->out) (void)hipFree(c->out);
+        SERBF_HIP(hipMalloc((void **)&c->out, (size_t)count * sizeof(float)));
+        c->out_cap = count;
+    }
+    const int ntiles = (int)(count / 16);
+    hipLaunchKernelGGL(eval_kernel, dim3(launch_grid_waves(ntiles)), dim3(256), 0, c->stream, view(c), c->feat,
+                       first, ntiles, c->out);
+    SERBF_HIP(hipGetLastError());
+    SERBF_HIP(hipMemcpyAsync(o, c->out, (size_t)count * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SERBF_HIP(hipStreamSynchronize(c->stream));
+    return SERBF_OK;
+}
+
+int serbf_closure_features(serbf_engine *e, double t, double dt, float *feat, double *lap)
+{
+    SERBF_CHECK(e && feat && lap, SERBF_ERR_ARG, "closure_features: null");
+    SERBF_CHECK(e->have_state && e->have_lap && e->s64.WL1, SERBF_ERR_STATE,
+                "closure_features: needs a state and config-mode weights with hv_order >= 3");
+    SERBF_HIP(hipSetDevice(e->device));
+    int rc = ensure_hloc(e);
+    if (rc) return rc;
+    double ts[4 * kTimeScalars] = {0};
+    time_scalars_public(e->cfg, t, ts);
+    const size_t nsm = (size_t)e->N * e->ntr * e->cfg.Nv;
+    double *d_ts = nullptr, *d_lap = nullptr;
+    float *d_feat = nullptr;
+    SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
+    SERBF_HIP(hipMalloc((void **)&d_lap, nsm * sizeof(double)));
+    SERBF_HIP(hipMalloc((void **)&d_feat, nsm * 16 * sizeof(float)));
+    SERBF_HIP(hipMemcpyAsync(d_ts, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
+    const FeatArgs<double> a = feat_args<double>(e, e->s64, e->s64.qT, dt, d_ts, e->d_base + 1, 0);
+    const long ncol = (long)e->N * e->ntr;
+    const long waves = e->S == 16 ? (ncol + 3) / 4 : ncol;
+    const dim3 grid((unsigned)((waves + 3) / 4));
+    if (e->S == 16) hipLaunchKernelGGL((features_kernel<double, 16>), grid, dim3(256), 0, e->stream, a, d_feat, d_lap);
+    else hipLaunchKernelGGL((features_kernel<double, 64>), grid, dim3(256), 0, e->stream, a, d_feat, d_lap);
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess) {
+        (void)hipMemcpyAsync(feat, d_feat, nsm * 16 * sizeof(float), hipMemcpyDeviceToHost, e->stream);
+        (void)hipMemcpyAsync(lap, d_lap, nsm * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+        le = hipStreamSynchronize(e->stream);
+    }
+    (void)hipFree(d_ts);
+    (void)hipFree(d_lap);
+    (void)hipFree(d_feat);
+    SERBF_HIP(le);
+    return SERBF_OK;
+}
+
+int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_max)
+{
+    SERBF_CHECK(e, SERBF_ERR_ARG, "closure_attach: null engine");
+    SERBF_HIP(hipSetDevice(e->device));
+    if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
+    if (!c) {
+        e->closure = nullptr;
+        e->hv_mode = 0;
+        return SERBF_OK;
+    }
+    SERBF_CHECK(every >= 1 && every <= 8 && s_max > 0.0, SERBF_ERR_ARG, "closure_attach: every 1..8, s_max > 0");
+    SERBF_CHECK(c->have_params, SERBF_ERR_STATE, "closure_attach: closure has no parameters");
+    SERBF_CHECK(c->device == e->device, SERBF_ERR_ARG, "closure_attach: closure and engine on different devices");
+    SERBF_CHECK(e->have_state && e->have_lap && e->s64.WL1 && e->s32.WL1, SERBF_ERR_STATE,
+                "closure_attach: needs a state and config-mode weights with hv_order >= 3");
+    SERBF_CHECK(e->cfg.hv_order > 0, SERBF_ERR_STATE, "closure_attach: hyperviscosity order is 0");
+    int rc = ensure_hloc(e);
+    if (!rc) rc = alloc_gamma(e);
+    if (rc) return rc;
+    SERBF_HIP(hipStreamSynchronize(c->stream));
+    e->closure = c;
+    e->closure_every = every;
+    e->closure_smax = s_max;
+    e->hv_mode = 1;
+    return SERBF_OK;
+}
+
+int serbf_closure_gamma(serbf_engine *e, int precision, double t, double dt, double *gamma)
+{
+    SERBF_CHECK(e && gamma && e->closure, SERBF_ERR_STATE, "closure_gamma: no closure attached");
+    SERBF_HIP(hipSetDevice(e->device));
+    double ts[4 * kTimeScalars] = {0};
+    time_scalars_public(e->cfg, t, ts);
+    const int S = e->S, Nv = e->cfg.Nv;
+    const size_t cols = (size_t)e->N * e->ntr, P = cols * S;
+    int rc = SERBF_OK;
+    if (precision == SERBF_F32) {
+        float tsf[4 * kTimeScalars];
+        for (int i = 0; i < 4 * kTimeScalars; i++) tsf[i] = (float)ts[i];
+        float *d_ts = nullptr;
+        SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(tsf)));
+        SERBF_HIP(hipMemcpyAsync(d_ts, tsf, sizeof(tsf), hipMemcpyHostToDevice, e->stream));
+        const FeatArgs<float> a = feat_args<float>(e, e->s32, e->s32.qT, dt, d_ts, e->d_base + 1, 0);
+        rc = launch_apply<float>(e, a, e->s32.gamma);
+        if (!rc) {
+            std::vector<float> g(P);
+            SERBF_HIP(hipMemcpyAsync(g.data(), e->s32.gamma, P * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+            SERBF_HIP(hipStreamSynchronize(e->stream));
+            for (size_t col = 0; col < cols; col++)
+                for (int v = 0; v < Nv; v++) gamma[col * Nv + v] = g[col * S + v];
+        }
+        (void)hipFree(d_ts);
+    } else {
+        double *d_ts = nullptr;
+        SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
+        SERBF_HIP(hipMemcpyAsync(d_ts, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
+        const FeatArgs<double> a = feat_args<double>(e, e->s64, e->s64.qT, dt, d_ts, e->d_base + 1, 0);
+        rc = launch_apply<double>(e, a, e->s64.gamma);
+        if (!rc) {
+            SERBF_HIP(hipMemcpy2DAsync(gamma, Nv * sizeof(double), e->s64.gamma, S * sizeof(double),
+                                       Nv * sizeof(double), cols, hipMemcpyDeviceToHost, e->stream));
+            SERBF_HIP(hipStreamSynchronize(e->stream));
+        }
+        (void)hipFree(d_ts);
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -54,6 +54,7 @@ struct StateBuffers {
     T *vW = nullptr;      // [S][kMaxNv]
     T *tscal = nullptr;   // [steps][4 stages][kTimeScalars]
     T *gamma = nullptr;   // per-point hyperviscosity coefficient [N][ntr][S] (closure)
+    T *WL1 = nullptr, *WL2 = nullptr;  // Delta, Delta^2 weights [N][n] (closure features)
 };
 
 struct GraphCache {
@@ -90,6 +91,9 @@ struct serbf_engine {
     double h_vW[serbf::kMaxLevels * serbf::kMaxNv];
     // weights (FP64 master copy), solve status
     double *d_Wx = nullptr, *d_Wy = nullptr, *d_WL = nullptr;
+    // config mode: Delta and Delta^2 weights (closure features), filled by the weight solves
+    double *d_WL1 = nullptr, *d_WL2 = nullptr;
+    bool have_lap = false;
     int *d_status = nullptr;
     bool have_weights = false;
     // time loop buffers
@@ -121,6 +125,11 @@ struct serbf_engine {
     // reusable scratch
     void *d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    // learned closure attached to the time loop (not owned)
+    serbf_closure *closure = nullptr;
+    int closure_every = 8;
+    double closure_smax = 8.0;
+    float *d_hloc = nullptr;   // nearest-neighbour chord per node (closure features)
 };
 
 namespace serbf {
@@ -144,4 +153,9 @@ int rk4_run(serbf_engine *e, int precision, int nsteps, double dt, double t0, bo
             double *ms, double *stage_ms);
 int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha,
                             const double *q_host, double *F_host);
+// the kTimeScalars flow time factors at time t (rk4.hip)
+void time_scalars_public(const serbf_config &c, double t, double *out);
+// closure (closure.hip): coefficients from the step-start state sb.qT into sb.gamma
+template <typename T>
+int closure_enqueue(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off);
 }  // namespace serbf

@@ -91,17 +91,22 @@ int serbf_create(serbf_engine **out, int device)
 
 static void free_nodes(serbf_engine *e)
 {
-    for (double *p : {e->d_x, e->d_y, e->d_z, e->d_lam, e->d_phi, e->d_D, e->d_Wx, e->d_Wy, e->d_WL, e->d_q0,
-                      e->d_wmass})
+    for (double *p : {e->d_x, e->d_y, e->d_z, e->d_lam, e->d_phi, e->d_D, e->d_Wx, e->d_Wy, e->d_WL, e->d_WL1,
+                      e->d_WL2, e->d_q0, e->d_wmass})
         if (p) (void)hipFree(p);
     for (int *p : {e->d_idx, e->d_status, e->d_vstart, e->d_tuni, e->d_tcount})
         if (p) (void)hipFree(p);
     if (e->d_lidx) (void)hipFree(e->d_lidx);
+    if (e->d_hloc) (void)hipFree(e->d_hloc);
+    e->d_hloc = nullptr;
+    e->closure = nullptr;
+    e->hv_mode = 0;
     e->d_tuni = e->d_tcount = nullptr;
     e->d_lidx = nullptr;
     e->have_plan = false;
-    e->d_x = e->d_y = e->d_z = e->d_lam = e->d_phi = e->d_D = e->d_Wx = e->d_Wy = e->d_WL = e->d_q0 =
-        e->d_wmass = nullptr;
+    e->d_x = e->d_y = e->d_z = e->d_lam = e->d_phi = e->d_D = e->d_Wx = e->d_Wy = e->d_WL = e->d_WL1 = e->d_WL2 =
+        e->d_q0 = e->d_wmass = nullptr;
+    e->have_lap = false;
     e->d_idx = e->d_status = e->d_vstart = nullptr;
     rk4_free(e);
     e->have_stencils = e->have_weights = e->have_state = e->f32_ready = false;
@@ -277,7 +282,7 @@ int serbf_get_weights(serbf_engine *e, double *Dx, double *Dy, double *L, int *s
 int serbf::alloc_weights(serbf_engine *e)
 {
     const size_t m = (size_t)e->N * e->n;
-    for (double **p : {&e->d_Wx, &e->d_Wy, &e->d_WL}) {
+    for (double **p : {&e->d_Wx, &e->d_Wy, &e->d_WL, &e->d_WL1, &e->d_WL2}) {
         if (*p) (void)hipFree(*p);
         SERBF_HIP(hipMalloc((void **)p, m * sizeof(double)));
     }
@@ -305,6 +310,18 @@ int serbf_set_weights(serbf_engine *e, const double *Dx, const double *Dy, const
     SERBF_HIP(hipMemsetAsync(e->d_status, 0, e->N * sizeof(int), e->stream));
     SERBF_HIP(hipStreamSynchronize(e->stream));
     e->have_weights = true;
+    e->have_lap = false;
+    return SERBF_OK;
+}
+
+int serbf_get_lap_weights(serbf_engine *e, double *L1, double *L2)
+{
+    SERBF_CHECK(e && e->have_weights && e->have_lap, SERBF_ERR_STATE,
+                "get_lap_weights: needs config-mode weights with hv_order >= 3 from serbf_weights");
+    const size_t m = (size_t)e->N * e->n;
+    if (L1) SERBF_HIP(hipMemcpyAsync(L1, e->d_WL1, m * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    if (L2) SERBF_HIP(hipMemcpyAsync(L2, e->d_WL2, m * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
     return SERBF_OK;
 }
 

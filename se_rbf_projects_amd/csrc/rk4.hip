@@ -544,6 +544,10 @@ int alloc_state(serbf_engine *e, StateBuffers<T> &sb)
         SERBF_HIP(hipMalloc((void **)&sb.Wx, (size_t)e->N * e->n * sizeof(T)));
         SERBF_HIP(hipMalloc((void **)&sb.Wy, (size_t)e->N * e->n * sizeof(T)));
         SERBF_HIP(hipMalloc((void **)&sb.WL, (size_t)e->N * e->n * sizeof(T)));
+        if (e->have_lap) {
+            SERBF_HIP(hipMalloc((void **)&sb.WL1, (size_t)e->N * e->n * sizeof(T)));
+            SERBF_HIP(hipMalloc((void **)&sb.WL2, (size_t)e->N * e->n * sizeof(T)));
+        }
         SERBF_HIP(hipMalloc((void **)&sb.basis, (size_t)e->N * kBasis * sizeof(T)));
         SERBF_HIP(hipMalloc((void **)&sb.lev, 4 * e->S * sizeof(T)));
         SERBF_HIP(hipMalloc((void **)&sb.vW, e->S * kMaxNv * sizeof(T)));
@@ -554,7 +558,8 @@ int alloc_state(serbf_engine *e, StateBuffers<T> &sb)
 template <typename T>
 void free_state(StateBuffers<T> &sb)
 {
-    for (T *b : {sb.qT, sb.qA, sb.qB, sb.F, sb.Wx, sb.Wy, sb.WL, sb.basis, sb.lev, sb.vW, sb.tscal, sb.gamma})
+    for (T *b : {sb.qT, sb.qA, sb.qB, sb.F, sb.Wx, sb.Wy, sb.WL, sb.basis, sb.lev, sb.vW, sb.tscal, sb.gamma, sb.WL1,
+                 sb.WL2})
         if (b) (void)hipFree(b);
     sb = StateBuffers<T>();
 }
@@ -568,12 +573,20 @@ int fill_tables(serbf_engine *e, StateBuffers<T> &sb, const double *basis64)
         SERBF_HIP(hipMemcpyAsync(sb.Wx, e->d_Wx, nw * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
         SERBF_HIP(hipMemcpyAsync(sb.Wy, e->d_Wy, nw * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
         SERBF_HIP(hipMemcpyAsync(sb.WL, e->d_WL, nw * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+        if (sb.WL1) {
+            SERBF_HIP(hipMemcpyAsync(sb.WL1, e->d_WL1, nw * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+            SERBF_HIP(hipMemcpyAsync(sb.WL2, e->d_WL2, nw * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+        }
         SERBF_HIP(hipMemcpyAsync(sb.basis, basis64, (size_t)e->N * kBasis * sizeof(double),
                                  hipMemcpyDeviceToDevice, e->stream));
     } else {
         hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream, nw, e->d_Wx, sb.Wx);
         hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream, nw, e->d_Wy, sb.Wy);
         hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream, nw, e->d_WL, sb.WL);
+        if (sb.WL1) {
+            hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream, nw, e->d_WL1, sb.WL1);
+            hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream, nw, e->d_WL2, sb.WL2);
+        }
         hipLaunchKernelGGL(convert_kernel<T>, dim3(blocks), dim3(thr), 0, e->stream,
                            (size_t)e->N * kBasis, basis64, sb.basis);
         SERBF_HIP(hipGetLastError());
@@ -749,6 +762,7 @@ int enqueue_step(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off)
     a.qT = sb.qT;
     a.F = sb.F;
     int rc;
+    if (e->closure && step_off % e->closure_every == 0 && (rc = closure_enqueue<T>(e, sb, dt, step_off))) return rc;
     a.qin = sb.qT; a.qout = sb.qB;
     if ((rc = launch_stage<T>(e, 1, hv1, a))) return rc;
     a.qin = sb.qB; a.qout = sb.qA;
@@ -772,7 +786,8 @@ int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, doubl
     SERBF_HIP(hipMemcpyAsync(e->d_base, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
     // the graph of kGraphSteps steps reads the step counter and bumps it at the end
     const void *key = (const void *)sb.tscal;
-    const int hvk = (hv_enabled(e) ? 1 : 0) + 2 * e->hv_mode + 4 * e->hv_split + 8 * e->kver + 32 * e->unroll;
+    const int hvk = (hv_enabled(e) ? 1 : 0) + 2 * e->hv_mode + 4 * e->hv_split + 8 * e->kver + 32 * e->unroll +
+                    1024 * (e->closure ? e->closure_every : 0);
     if (nsteps >= kGraphSteps && (!e->graph.exec || e->graph.precision != precision || e->graph.key != key ||
                                   e->graph.dt != dt || e->graph.hv != hvk)) {
         if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
@@ -809,6 +824,8 @@ int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, doubl
 }
 
 }  // namespace
+
+void time_scalars_public(const serbf_config &c, double t, double *out) { time_scalars(c, t, out); }
 
 // Allocate the FP64 time-loop buffers and fill the per-node/per-level tables.
 int rk4_prepare(serbf_engine *e)

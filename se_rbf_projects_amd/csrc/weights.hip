@@ -37,11 +37,23 @@ struct WArgs {
     const double *x, *y, *z;
     const int *idx;
     double *Wx, *Wy, *WL;
+    double *WL1, *WL2;   // config mode: Delta and Delta^2 weights from the Delta^k system (may be null)
     int *status;
     int *counter;
 };
 
-__device__ __forceinline__ double phs_pow(double r, int k) { return pow(r, (double)k); }
+// x^k for a small integer k >= 0 by repeated squaring (the PHS orders and monomial degrees)
+__device__ __forceinline__ double ipow(double x, int k)
+{
+    double r = 1.0, b = x;
+    while (k > 0) {
+        if (k & 1) r *= b;
+        b *= b;
+        k >>= 1;
+    }
+    return r;
+}
+__device__ __forceinline__ double phs_pow(double r, int k) { return k >= 0 ? ipow(r, k) : pow(r, (double)k); }
 
 // 2D Delta^k of the Gaussian exp(-(eps r)^2) by the Laguerre recursion (rbffd_DMs.c:314-328)
 __device__ double lap_gauss(double r, double eps, int k)
@@ -98,7 +110,7 @@ __device__ __forceinline__ double entry(int i, int j, int n, int m, int kind, co
     int p = 0;
     while ((p + 1) * (p + 2) / 2 <= t) p++;
     const int q = t - p * (p + 1) / 2;
-    return pow(X[node], (double)(p - q)) * pow(X[MAXN + node], (double)q);
+    return ipow(X[node], p - q) * ipow(X[MAXN + node], q);
 }
 
 template <int MAXM>
@@ -146,8 +158,23 @@ __device__ int factor_solve(double *A, double *X, double *B0, double *XS, int *p
             if (i > k && i < m) {
                 const double l = A[i * LD + k] / pv;
                 A[i * LD + k] = l;
-                if (l != 0.0)
-                    for (int j = k + 1; j < m + nrhs; j++) A[i * LD + j] -= l * A[k * LD + j];
+                if (l != 0.0) {
+                    // groups of 4 independent loads before the stores: the row update is a
+                    // latency chain through LDS otherwise
+                    double *ri = A + i * LD;
+                    const double *rk = A + k * LD;
+                    const int mm = m + nrhs;
+                    int j = k + 1;
+                    for (; j + 4 <= mm; j += 4) {
+                        const double a0 = ri[j], a1 = ri[j + 1], a2 = ri[j + 2], a3 = ri[j + 3];
+                        const double b0 = rk[j], b1 = rk[j + 1], b2 = rk[j + 2], b3 = rk[j + 3];
+                        ri[j] = a0 - l * b0;
+                        ri[j + 1] = a1 - l * b1;
+                        ri[j + 2] = a2 - l * b2;
+                        ri[j + 3] = a3 - l * b3;
+                    }
+                    for (; j < mm; j++) ri[j] -= l * rk[j];
+                }
             }
         }
         wave_sync();
@@ -247,19 +274,32 @@ __device__ void assemble(double *A, double *X, double *B0, int n, int m, int nrh
             const double r = sqrt(dx * dx + dy * dy + dz * dz);
             if (kind == 0) {
                 const int k = a.phs;
-                b[0] = k * dx * pow(r, (double)(k - 2));
-                b[1] = k * dy * pow(r, (double)(k - 2));
+                b[0] = k * dx * phs_pow(r, k - 2);
+                b[1] = k * dy * phs_pow(r, k - 2);
             } else if (a.ref_mode) {
                 b[0] = lap_gauss(r, a.eps, a.hv);
             } else {
+                // Delta^j r^K = prod_{i<j} (K - 2i)^2 r^(K - 2j) (2D radial Laplacian), j = k, 1, 2
+                const double K = (double)a.hvphs;
                 double c = 1.0;
-                for (int jj = 0; jj < a.hv; jj++) c *= (double)(a.hvphs - 2 * jj) * (double)(a.hvphs - 2 * jj);
-                b[0] = c * pow(r, (double)(a.hvphs - 2 * a.hv));
+                for (int jj = 0; jj < a.hv; jj++) c *= (K - 2 * jj) * (K - 2 * jj);
+                b[0] = c * phs_pow(r, a.hvphs - 2 * a.hv);
+                b[1] = K * K * phs_pow(r, a.hvphs - 2);
+                b[2] = K * K * (K - 2.0) * (K - 2.0) * phs_pow(r, a.hvphs - 4);
             }
         } else if (kind == 0) {
             // d/dx' of x' and d/dy' of y' at the origin of the tangent plane
             b[0] = (i - n == 1) ? 1.0 : 0.0;
             b[1] = (i - n == 2) ? 1.0 : 0.0;
+        } else if (!a.ref_mode) {
+            // Delta and Delta^2 of the monomial x^(p-q) y^q at the origin (Delta^k, k >= 4, of a
+            // monomial of degree < 2k vanishes there)
+            const int t = i - n;
+            int p = 0;
+            while ((p + 1) * (p + 2) / 2 <= t) p++;
+            const int q = t - p * (p + 1) / 2;
+            b[1] = (p == 2 && (q == 0 || q == 2)) ? 2.0 : 0.0;
+            b[2] = p == 4 ? ((q == 0 || q == 4) ? 24.0 : (q == 2 ? 8.0 : 0.0)) : 0.0;
         }
         for (int c = 0; c < nrhs; c++) {
             A[i * LD + m + c] = b[c];
@@ -331,11 +371,19 @@ __global__ void __launch_bounds__(64) weights_kernel(WArgs a)
         wave_sync();
         // Delta^k system
         if (a.hv > 0) {
-            assemble<MAXM>(A, X, B0, n, m2, 1, 1, a, lane);
-            int bad = factor_solve<MAXM>(A, X, B0, XS, piv, n, m2, 1, 1, a, lane);
+            const int nr = a.WL1 ? 3 : 1;
+            assemble<MAXM>(A, X, B0, n, m2, nr, 1, a, lane);
+            int bad = factor_solve<MAXM>(A, X, B0, XS, piv, n, m2, nr, 1, a, lane);
             if (bad && !st) st = 2;
             const double s2k = pow(scale, 2.0 * a.hv);
             for (int s = lane; s < n; s += 64) a.WL[(size_t)node * n + s] = bad ? 0.0 : XS[s * 3] / s2k;
+            if (a.WL1) {
+                const double s2 = scale * scale, s4 = s2 * s2;
+                for (int s = lane; s < n; s += 64) {
+                    a.WL1[(size_t)node * n + s] = bad ? 0.0 : XS[s * 3 + 1] / s2;
+                    a.WL2[(size_t)node * n + s] = bad ? 0.0 : XS[s * 3 + 2] / s4;
+                }
+            }
         } else {
             for (int s = lane; s < n; s += 64) a.WL[(size_t)node * n + s] = 0.0;
         }
@@ -380,6 +428,9 @@ int weights_run(serbf_engine *e)
     a.x = e->d_x; a.y = e->d_y; a.z = e->d_z;
     a.idx = e->d_idx;
     a.Wx = e->d_Wx; a.Wy = e->d_Wy; a.WL = e->d_WL;
+    const bool lap = c.mode == SERBF_MODE_CONFIG && c.hv_order >= 3;
+    a.WL1 = lap ? e->d_WL1 : nullptr;
+    a.WL2 = lap ? e->d_WL2 : nullptr;
     a.status = e->d_status;
     a.counter = (int *)e->d_scratch;
     if (m <= 48) rc = launch_weights<48>(e, a);
@@ -390,6 +441,7 @@ int weights_run(serbf_engine *e)
     SERBF_HIP(hipMemcpyAsync(st.data(), e->d_status, e->N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     SERBF_HIP(hipStreamSynchronize(e->stream));
     e->have_weights = true;
+    e->have_lap = lap;
     rk4_free(e);
     e->have_state = false;
     for (int s : st)

@@ -69,6 +69,18 @@ def lib():
             "serbf_set_stage_unroll": ([vp, i], i),
             "serbf_set_tile_chunk": ([vp, i], i),
             "serbf_tile_info": ([vp, vp, vp], i),
+            "serbf_get_lap_weights": ([vp, vp, vp], i),
+            "serbf_closure_create": ([C.POINTER(vp), i], i),
+            "serbf_closure_destroy": ([vp], None),
+            "serbf_closure_set_params": ([vp, vp, vp], i),
+            "serbf_closure_get_params": ([vp, vp], i),
+            "serbf_closure_set_data": ([vp, C.c_long, vp, vp, vp], i),
+            "serbf_closure_grad": ([vp, C.c_long, C.c_long, vp], i),
+            "serbf_closure_train": ([vp, C.c_long, C.c_long, i, i, d, vp], i),
+            "serbf_closure_eval": ([vp, C.c_long, C.c_long, vp], i),
+            "serbf_closure_features": ([vp, d, d, vp, vp], i),
+            "serbf_closure_attach": ([vp, vp, i, d], i),
+            "serbf_closure_gamma": ([vp, i, d, d, vp], i),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -120,6 +132,65 @@ def initial_condition(kind, cfg, xyz, seed=0):
     _check(lib().serbf_initial_condition(kind, C.byref(cfg), len(x), _p(x), _p(y), _p(z), int(seed), _p(q)),
            "initial_condition")
     return q
+
+
+class Closure:
+    """Learned hyperviscosity closure: parameters, training set and Adam state on one GPU."""
+
+    NPARAMS = 4929
+
+    def __init__(self, device=0):
+        self._c = C.c_void_p()
+        _check(lib().serbf_closure_create(C.byref(self._c), int(device)), "closure_create")
+
+    def close(self):
+        if self._c:
+            lib().serbf_closure_destroy(self._c)
+            self._c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params, norm):
+        p = np.ascontiguousarray(params, np.float32)
+        nm = np.ascontiguousarray(norm, np.float32)
+        assert p.size == self.NPARAMS and nm.size == 32
+        _check(lib().serbf_closure_set_params(self._c, _p(p), _p(nm)), "closure_set_params")
+
+    def params(self):
+        p = np.empty(self.NPARAMS, np.float32)
+        _check(lib().serbf_closure_get_params(self._c, _p(p)), "closure_get_params")
+        return p
+
+    def set_data(self, feat, A, R):
+        self._feat = np.ascontiguousarray(feat, np.float32)
+        self._A = np.ascontiguousarray(A, np.float32)
+        self._R = np.ascontiguousarray(R, np.float32)
+        n = len(self._A)
+        assert self._feat.shape == (n, 16) and self._R.shape == (n,)
+        _check(lib().serbf_closure_set_data(self._c, n, _p(self._feat), _p(self._A), _p(self._R)), "closure_set_data")
+        self.n = n
+
+    def grad(self, first, count):
+        g = np.empty(self.NPARAMS + 1, np.float32)
+        _check(lib().serbf_closure_grad(self._c, int(first), int(count), _p(g)), "closure_grad")
+        return g[:-1], float(g[-1])
+
+    def train(self, steps, batch, lr, first=0, count=None):
+        """Adam steps over batches of the sample range [first, first + count) (default: all)."""
+        count = self.n - first if count is None else count
+        losses = np.empty(max(steps, 1), np.float32)
+        _check(lib().serbf_closure_train(self._c, int(first), int(count), int(steps), int(batch), float(lr),
+                                         _p(losses)), "closure_train")
+        return losses[:steps]
+
+    def eval(self, first, count):
+        o = np.empty(count, np.float32)
+        _check(lib().serbf_closure_eval(self._c, int(first), int(count), _p(o)), "closure_eval")
+        return o
 
 
 class Engine:
@@ -250,6 +321,30 @@ class Engine:
         ms = C.c_double()
         _check(lib().serbf_stream_copy(self._e, int(nbytes), int(reps), C.byref(ms)), "stream_copy")
         return ms.value
+
+    def lap_weights(self):
+        """Delta and Delta^2 weights [N, n] (config mode, hv_order >= 3)."""
+        L1, L2 = (np.empty((self.N, self.n)) for _ in range(2))
+        _check(lib().serbf_get_lap_weights(self._e, _p(L1), _p(L2)), "get_lap_weights")
+        return L1, L2
+
+    def closure_features(self, t, dt):
+        """Closure features of the FP64 state: feat [N*ntr*Nv, 16] float32, lap [N*ntr*Nv]."""
+        m = self.N * self.ntr * self.cfg.Nv
+        feat = np.empty((m, 16), np.float32)
+        lap = np.empty(m)
+        _check(lib().serbf_closure_features(self._e, float(t), float(dt), _p(feat), _p(lap)), "closure_features")
+        return feat, lap
+
+    def attach_closure(self, closure, every=8, s_max=8.0):
+        self._closure = closure
+        _check(lib().serbf_closure_attach(self._e, closure._c if closure is not None else None, int(every),
+                                          float(s_max)), "closure_attach")
+
+    def closure_gamma(self, precision, t, dt):
+        g = np.empty((self.N, self.ntr, self.cfg.Nv)) if self.ntr > 1 else np.empty((self.N, self.cfg.Nv))
+        _check(lib().serbf_closure_gamma(self._e, int(precision), float(t), float(dt), _p(g)), "closure_gamma")
+        return g
 
     def sync_f32(self):
         _check(lib().serbf_sync_f32_from_f64(self._e), "sync_f32")

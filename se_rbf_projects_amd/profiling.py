@@ -2,14 +2,18 @@
 
     python -m se_rbf_projects_amd.profiling trace <dir-with-rocpd-db>
     python -m se_rbf_projects_amd.profiling pmc <dir> [<dir> ...]
+    python -m se_rbf_projects_amd.profiling traffic <fetch-dir> <write-dir> <copy-MiB> [out.json]
 
 `trace` prints per-kernel call counts and average durations from a
 `rocprofv3 --kernel-trace --stats` run; `pmc` prints per-kernel averages of the
-counters of one or more `rocprofv3 --pmc ...` runs.  Both read the rocpd SQLite
+counters of one or more `rocprofv3 --pmc ...` runs; `traffic` turns a FETCH_SIZE run and
+a WRITE_SIZE run into HBM bytes per launch, each counter scaled by what it reads for the
+16-byte stream-copy kernel of the same run (which moves copy-MiB in each direction).  Both read the rocpd SQLite
 database rocprofv3 writes (the `kernels` / `counters_collection` views).
 """
 import collections
 import glob
+import json
 import os
 import sqlite3
 import sys
@@ -44,6 +48,25 @@ def counter_stats(paths):
     return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
 
 
+def traffic(fetch_dir, write_dir, copy_mib):
+    """{kernel: {fetch_KiB, write_KiB, hbm_bytes_per_launch}} with the counters calibrated on
+    stream_copy_kernel, and the calibration factors."""
+    f = counter_stats([fetch_dir])
+    w = counter_stats([write_dir])
+    copy = "stream_copy_kernel"
+    nbytes = copy_mib * 1024 * 1024
+    kf = nbytes / (f[copy]["FETCH_SIZE"] * 1024.0)
+    kw = nbytes / (w[copy]["WRITE_SIZE"] * 1024.0)
+    out = {}
+    for k in sorted(set(f) & set(w)):
+        fk, wk = f[k].get("FETCH_SIZE"), w[k].get("WRITE_SIZE")
+        if fk is None or wk is None:
+            continue
+        out[k] = {"fetch_KiB": fk, "write_KiB": wk, "hbm_bytes_per_launch": kf * fk * 1024.0 + kw * wk * 1024.0}
+    return {"calibration": {"copy_bytes_each_way": nbytes, "fetch_factor": kf, "write_factor": kw},
+            "kernels": out}
+
+
 def main(argv):
     if len(argv) < 2:
         print(__doc__)
@@ -55,6 +78,13 @@ def main(argv):
         print("|---|---|---|---|---|")
         for k, (c, a, t) in sorted(st.items(), key=lambda kv: -kv[1][2]):
             print(f"| `{k}` | {c} | {a:.1f} | {t:.0f} | {100 * t / tot:.1f}% |")
+    elif argv[0] == "traffic":
+        d = traffic(argv[1], argv[2], float(argv[3]))
+        txt = json.dumps(d, indent=1)
+        if len(argv) > 4:
+            with open(argv[4], "w") as fh:
+                fh.write(txt + "\n")
+        print(txt)
     else:
         st = counter_stats(argv[1:])
         names = sorted({c for d in st.values() for c in d})

@@ -28,6 +28,18 @@ def lib():
         _lib.orc_householder.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, c_double_p]
         _lib.orc_dgesv.restype = C.c_int
         _lib.orc_weights.restype = C.c_int
+        _lib.orc_weights_lap.restype = C.c_int
+        _lib.orc_bf16.argtypes = [C.c_float]
+        _lib.orc_bf16.restype = C.c_float
+        _lib.orc_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        _lib.orc_mlp_forward.restype = C.c_double
+        _lib.orc_mlp_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int, C.c_void_p]
+        _lib.orc_mlp_grad.restype = C.c_double
+        _lib.orc_adam.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                  C.c_double, C.c_double, C.c_double]
+        _lib.orc_closure_features.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 11 + [C.c_int, C.c_int,
+                                              C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
         _lib.orc_vertical.restype = C.c_int
         _lib.orc_ref_node_weights.restype = C.c_int
         for name in ("orc_velocity", "orc_velocity_cart"):
@@ -101,6 +113,19 @@ def weights(cfg, xyz, idx):
     return Dx, Dy, L, st
 
 
+def weights_lap(cfg, xyz, idx):
+    """Config-mode weights plus the Delta (L1) and Delta^2 (L2) weights of the Delta^k system."""
+    xyz = _c(xyz, np.float64)
+    idx = _c(idx, np.int32)
+    N, n = idx.shape
+    x, y, z = (np.ascontiguousarray(xyz[:, i]) for i in range(3))
+    Dx, Dy, L, L1, L2 = (np.zeros((N, n)) for _ in range(5))
+    st = np.zeros(N, np.int32)
+    lib().orc_weights_lap(C.byref(cfg), C.c_int(N), as_ptr(x), as_ptr(y), as_ptr(z), as_ptr(idx, C.c_int),
+                          as_ptr(Dx), as_ptr(Dy), as_ptr(L), as_ptr(L1), as_ptr(L2), as_ptr(st, C.c_int))
+    return Dx, Dy, L, L1, L2, st
+
+
 def vertical(cfg):
     nvmax = 7 if cfg.mode == 0 else cfg.nv
     start = np.zeros(cfg.Nv, np.int32)
@@ -162,3 +187,60 @@ class Problem:
         qa = _c(q, np.float64)
         lib().orc_norms(C.byref(self.cfg), self.N, qr.ctypes.data, qa.ctypes.data, out.ctypes.data)
         return out
+
+
+# ---- learned closure ----
+NPARAMS = 4929
+
+
+def bf16(x):
+    return float(lib().orc_bf16(float(x)))
+
+
+def mlp_forward(params, norm, feat, emulate=True):
+    """Network output o for each row of feat [m, 16] (float32)."""
+    p = _c(params, np.float32)
+    nm = _c(norm, np.float32)
+    f = _c(feat, np.float32).reshape(-1, 16)
+    return np.array([lib().orc_mlp_forward(as_ptr(p, C.c_float), as_ptr(nm, C.c_float), as_ptr(f[i], C.c_float),
+                                           int(emulate), None, None) for i in range(len(f))])
+
+
+def mlp_grad(params, norm, feat, A, R, emulate=True):
+    p = _c(params, np.float32)
+    nm = _c(norm, np.float32)
+    f = _c(feat, np.float32)
+    A = _c(A, np.float32)
+    R = _c(R, np.float32)
+    g = np.zeros(NPARAMS)
+    loss = lib().orc_mlp_grad(as_ptr(p, C.c_float), as_ptr(nm, C.c_float), len(A), as_ptr(f, C.c_float),
+                              as_ptr(A, C.c_float), as_ptr(R, C.c_float), int(emulate), as_ptr(g))
+    return g, loss
+
+
+def adam(params, m, v, g, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    p = np.array(params, np.float32)
+    m = np.array(m, np.float64)
+    v = np.array(v, np.float64)
+    g = _c(g, np.float64)
+    lib().orc_adam(len(p), as_ptr(p, C.c_float), as_ptr(m), as_ptr(v), as_ptr(g), int(step), lr, b1, b2, eps)
+    return p, m, v
+
+
+def closure_features(cfg, xyz, idx, Dx, Dy, L, L1, L2, vstart, vW, q, t, dt):
+    """Features [N*ntr*Nv, 16] and Delta^k q of q [N, ntr, Nv] (or [N, Nv])."""
+    xyz = _c(xyz, np.float64)
+    x, y, z = (np.ascontiguousarray(xyz[:, i]) for i in range(3))
+    N, n = idx.shape
+    q = _c(q, np.float64)
+    ntr = q.shape[1] if q.ndim == 3 else 1
+    m = N * ntr * cfg.Nv
+    feat = np.zeros((m, 16), np.float32)
+    lap = np.zeros(m)
+    arrs = [_c(a, np.float64) for a in (Dx, Dy, L, L1, L2)]
+    vs = _c(vstart, np.int32)
+    vw = _c(vW, np.float64)
+    lib().orc_closure_features(C.byref(cfg), N, as_ptr(x), as_ptr(y), as_ptr(z), as_ptr(_c(idx, np.int32), C.c_int),
+                               *[as_ptr(a) for a in arrs], as_ptr(vs, C.c_int), as_ptr(vw), vw.shape[1], ntr,
+                               as_ptr(q), float(t), float(dt), as_ptr(feat, C.c_float), as_ptr(lap))
+    return feat, lap

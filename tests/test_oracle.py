@@ -133,6 +133,55 @@ def test_weights_polynomial_reproduction_in_tangent_plane(golden):
         assert abs(L[p].sum()) < 1e-8 * np.max(np.abs(L[p]))
 
 
+@pytest.mark.parametrize("deg", [3, 4])
+def test_laplacian_weights_reproduce_polynomials(golden, deg):
+    """The Delta and Delta^2 weights of the PHS r^9 system apply the tangent-plane
+    Laplacian and bi-Laplacian exactly to every basis monomial x'^a y'^b (a + b <= deg):
+    Delta x^2 = Delta y^2 = 2, Delta^2 x^4 = Delta^2 y^4 = 24, Delta^2 x^2 y^2 = 8, others 0
+    at the centre.  A wrong sign, power or scale in the right-hand sides fails here."""
+    xyz = golden.icos(642)[:, :3]
+    cfg = config_A(642) if deg == 3 else config_deform(642, Nv=10, n=31, deg=4, nv=5)
+    idx, _ = O.knn(xyz, cfg.n)
+    Dx, Dy, L, L1, L2, st = O.weights_lap(cfg, xyz, idx)
+    assert not st.any()
+    lap = {(2, 0): 2.0, (0, 2): 2.0}
+    bilap = {(4, 0): 24.0, (0, 4): 24.0, (2, 2): 8.0}
+    for p in (0, 17, 300, 641):
+        H = O.householder(xyz[p], fix_pole=True)
+        Xp = (H @ xyz[idx[p]].T).T
+        c = Xp[0]
+        for a in range(deg + 1):
+            for b in range(deg + 1 - a):
+                v = (Xp[:, 0] - c[0]) ** a * (Xp[:, 1] - c[1]) ** b
+                s1 = np.abs(L1[p]) @ np.abs(v) + 1
+                s2 = np.abs(L2[p]) @ np.abs(v) + 1
+                assert abs(L1[p] @ v - lap.get((a, b), 0.0)) < 1e-9 * s1, (p, a, b)
+                assert abs(L2[p] @ v - bilap.get((a, b), 0.0)) < 1e-9 * s2, (p, a, b)
+    # the generic entry point returns the same gradient and Delta^k weights
+    Dx0, Dy0, L0, _ = O.weights(cfg, xyz, idx)
+    assert np.array_equal(Dx0, Dx) and np.array_equal(L0, L)
+
+
+def test_laplacian_weights_on_spherical_harmonic(golden):
+    """On the sphere, Delta Y_l = -l(l+1) Y_l; the tangent-plane Laplacian weights approach it
+    as h -> 0 (second order).  Y = z (l = 1) and Y = 3z^2 - 1 (l = 2), unit sphere."""
+    errs = []
+    for N in (2562, 10242):
+        xyz = golden.icos(N)[:, :3]
+        cfg = config_deform(N, Nv=10, n=31, deg=4, nv=5)
+        idx, _ = O.knn(xyz, cfg.n)
+        _, _, _, L1, _, st = O.weights_lap(cfg, xyz, idx)
+        assert not st.any()
+        worst = 0.0
+        for f, lam in ((lambda p: p[:, 2], -2.0), (lambda p: 3 * p[:, 2] ** 2 - 1, -6.0)):
+            fv = f(xyz)
+            lap = np.einsum("ij,ij->i", L1, fv[idx])
+            worst = max(worst, np.max(np.abs(lap - lam * fv)))
+        errs.append(worst)
+    # the tangent-plane operator misses the curvature term O(h^2) relative; it must shrink ~4x
+    assert errs[1] < 0.45 * errs[0] and errs[1] < 0.05, errs
+
+
 def test_reference_gaussian_laplacian_closed_form():
     """Laguerre recursion of L_garbf (rbffd_DMs.c:314-328) = 2D Delta^k of exp(-(eps r)^2)."""
     import sympy as sp
