@@ -34,7 +34,7 @@ extern "C" {
                                     7-point FD1 vertical with zero ghosts, H = I at the
                                     south pole (rotated vertical velocity sign kept) */
 #define SERBF_MODE_CONFIG 1      /* PHS r^k + degree-d tangent-plane monomials on radius-
-                                    scaled stencils, PHS Delta^k system, 1D RBF-FD vertical
+                                    scaled stencils, Gaussian or PHS Delta^k system, 1D RBF-FD vertical
                                     stencils (one-sided at the boundaries), radial
                                     velocity taken as w everywhere */
 
@@ -59,8 +59,10 @@ typedef struct serbf_config {
     int phs_order;          /* odd PHS order of the gradient system */
     int poly_deg;           /* tangent-plane polynomial degree (reference mode: 5) */
     int hv_order;           /* k of Delta^k (0: no hyperviscosity weights) */
-    int hv_phs_order;       /* config mode: odd PHS order of the Delta^k system, >= 2k+1 */
-    double hv_eps;          /* reference mode: Gaussian shape parameter epsilon */
+    int hv_phs_order;       /* config mode: odd PHS order of the Delta^k system (>= 2k+1, with the
+                               gradient system's monomials), or 0: Gaussian + constant Delta^k
+                               system with shape parameter hv_eps (the reference's form) */
+    double hv_eps;          /* Gaussian shape parameter epsilon of the unit sphere */
 
     /* vertical operator (config mode) */
     int nv;                 /* vertical stencil size (odd, <= 9) */
@@ -169,15 +171,17 @@ int serbf_tile_info(serbf_engine *e, int *ntiles, int *umax);
  * copy kernel); *ms_per_copy is the average launch time.  Reference for the
  * attainable HBM bandwidth and for calibrating the HBM byte counters. */
 int serbf_stream_copy(serbf_engine *e, size_t bytes, int reps, double *ms_per_copy);
-/* Delta and Delta^2 weights L1, L2 double[N*n] (config mode, hv_order >= 3): solved together
- * with the Delta^k weights from the same PHS r^K saddle system; used by the closure features. */
+/* Delta and Delta^2 weights L1, L2 double[N*n] (config mode, poly_deg >= 2): solved together
+ * with the gradient weights from the same PHS r^k + monomial saddle system (extra right-hand
+ * sides); used by the closure features. */
 int serbf_get_lap_weights(serbf_engine *e, double *L1, double *L2);
 
 /* ---- learned hyperviscosity closure ----
- * A per-point multiplier s = softplus(MLP(x)) of the hyperviscosity coefficient,
+ * A per-point multiplier s = min(softplus(MLP(x)), s_max) of the hyperviscosity coefficient,
  * c = hv_coef * s, with the MLP 10 -> 64 -> 64 -> 1 (tanh hidden layers) evaluated in BF16
  * on the matrix cores with FP32 accumulation.  s = 1 is the constant-coefficient method,
- * which is also the fallback wherever s is not finite or exceeds s_max.
+ * which is also the fallback wherever s is not finite.  s_max keeps c inside the stability
+ * region of the explicit RK4 step.
  *
  * Features of a point (node p, tracer, level v), with h the node's nearest-neighbour chord
  * (unit sphere) and asinh compression a(y) = asinh(8 y):
@@ -196,6 +200,8 @@ void serbf_closure_destroy(serbf_closure *c);
  * Resets the Adam moments. */
 int serbf_closure_set_params(serbf_closure *c, const float *params, const float *norm);
 int serbf_closure_get_params(serbf_closure *c, float *params);
+/* The clamp s_max of the multiplier in the training loss (default 16, > 1). */
+int serbf_closure_set_smax(serbf_closure *c, double s_max);
 /* Training set: n samples of features feat[n][16] (first 10 used, raw as above), model
  * column A[n] and target R[n].  The loss of a sample is ((s - 1) A - R)^2, so s = 1 scores
  * mean(R^2).  Samples are used in the given order (shuffle beforehand). */
@@ -214,7 +220,9 @@ int serbf_closure_eval(serbf_closure *c, long first, long count, float *o);
  * lap[...] = Delta^k q (unit sphere).  Needs config-mode weights with hv_order >= 3. */
 int serbf_closure_features(serbf_engine *e, double t, double dt, float *feat, double *lap);
 /* Use the closure in the engine's time loop: the coefficients are re-evaluated from the
- * step-start state every `every` steps (1..8).  c = NULL returns to the constant coefficient. */
+ * step-start state every `every` steps (1..8), with the multiplier clamped to s_max.
+ * c = NULL returns to the constant coefficient.  The engine keeps a pointer to c: detach
+ * before destroying it. */
 int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_max);
 /* Evaluate the attached closure once on the state of the given precision at time t (step dt)
  * and return the coefficients gamma double[N*ntr*Nv] (1/s). */

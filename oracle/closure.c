@@ -75,7 +75,7 @@ double orc_mlp_forward(const float *P, const float *norm, const float *feat, int
 }
 
 double orc_mlp_grad(const float *P, const float *norm, long ns, const float *feat, const float *A,
-                    const float *R, int emulate, double *grad)
+                    const float *R, int emulate, double s_max, double *grad)
 {
     memset(grad, 0, sizeof(double) * ORC_CLOSURE_PARAMS);
     double loss = 0.0;
@@ -84,9 +84,12 @@ double orc_mlp_grad(const float *P, const float *norm, long ns, const float *fea
         const float *fs = feat + s * 16;
         const double o = orc_mlp_forward(P, norm, fs, emulate, h1, h2);
         inputs(norm, fs, emulate, x);
-        const double e = (softplus(o) - 1.0) * A[s] - R[s];
+        /* s = min(softplus(o), s_max), no gradient through the clamp (s_max <= 0: no clamp) */
+        const double sp = softplus(o);
+        const int clamped = s_max > 0.0 && sp > s_max;
+        const double e = ((clamped ? s_max : sp) - 1.0) * A[s] - R[s];
         loss += e * e;
-        const double go = 2.0 * e * A[s] * sigmoid(o) / (double)ns;
+        const double go = clamped ? 0.0 : 2.0 * e * A[s] * sigmoid(o) / (double)ns;
         grad[OB3] += go;
         for (int n = 0; n < NH; n++) {
             grad[OW3 + n] += go * h2[n];

@@ -51,8 +51,8 @@ int orc_ref_node_weights(int n, const double *X /* 3 x n */, const double H[9], 
 int orc_weights(const serbf_config *cfg, int N, const double *x, const double *y,
                 const double *z, const int *idx, double *Dx, double *Dy, double *L,
                 int *status);
-/* Same, config mode also returning the Delta (L1) and Delta^2 (L2) weights of the
- * Delta^k system (PHS r^K + the same monomials); L1/L2 may be NULL. */
+/* Same, config mode also returning the Delta (L1) and Delta^2 (L2) weights solved with the
+ * gradient system (PHS r^k + the same monomials); L1/L2 may be NULL. */
 int orc_weights_lap(const serbf_config *cfg, int N, const double *x, const double *y,
                     const double *z, const int *idx, double *Dx, double *Dy, double *L,
                     double *L1, double *L2, int *status);
@@ -115,7 +115,7 @@ float orc_bf16(float x);
 double orc_mlp_forward(const float *params, const float *norm, const float *feat, int emulate,
                        double *h1, double *h2);
 double orc_mlp_grad(const float *params, const float *norm, long ns, const float *feat,
-                    const float *A, const float *R, int emulate, double *grad);
+                    const float *A, const float *R, int emulate, double s_max, double *grad);
 void orc_adam(int np, float *params, double *m, double *v, const double *g, int step, double lr,
               double b1, double b2, double eps);
 void orc_closure_features(const serbf_config *cfg, int N, const double *x, const double *y,

@@ -232,7 +232,7 @@ static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, con
     const int d = cfg->poly_deg, M = (d + 1) * (d + 2) / 2, m = n + M;
     double *xs = malloc(sizeof(double) * 3 * n);
     double *A = malloc(sizeof(double) * (size_t)m * m);
-    double *B = malloc(sizeof(double) * (size_t)m * 2);
+    double *B = malloc(sizeof(double) * (size_t)m * 4);
     double *A2 = malloc(sizeof(double) * (size_t)m * m);
     double *B2 = malloc(sizeof(double) * (size_t)m * 3);
     int *ipiv = malloc(sizeof(int) * m);
@@ -253,22 +253,32 @@ static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, con
         xs[j] /= s;
     const double *px = xs, *py = xs + n, *pz = xs + 2 * n;
     const int k = cfg->phs_order, K = cfg->hv_phs_order, kk = cfg->hv_order;
+    /* hv_phs_order == 0: Gaussian + constant Delta^k system (the reference's form,
+     * rbffd_DMs.c:188,223-228, L_garbf rbffd_DMs.c:314-328) with the shape parameter
+     * hv_eps of the unit sphere, i.e. eps * s on the scaled stencil */
+    const int gauss = kk > 0 && K == 0;
+    const double geps = cfg->hv_eps * s;
     const double cL = kk > 0 ? lap_k_phs_coef(K, kk) : 0.0;
     for (int i = 0; i < n; i++) {
         for (int j = 0; j < n; j++) {
             double dx = px[j] - px[i], dy = py[j] - py[i], dz = pz[j] - pz[i];
             double r = sqrt(dx * dx + dy * dy + dz * dz);
             A[(size_t)i * m + j] = pow(r, k);
-            A2[(size_t)i * m + j] = kk > 0 ? pow(r, K) : 0.0;
+            A2[(size_t)i * m + j] = kk > 0 ? (gauss ? ga(r, geps) : pow(r, K)) : 0.0;
         }
         double dx = -px[i], dy = -py[i], dz = -pz[i];
         double r = sqrt(dx * dx + dy * dy + dz * dz);
-        B[2 * i] = k * dx * pow(r, k - 2);
-        B[2 * i + 1] = k * dy * pow(r, k - 2);
+        /* d/dx', d/dy', Delta, Delta^2 of r^k at the centre */
+        B[4 * i] = k * dx * pow(r, k - 2);
+        B[4 * i + 1] = k * dy * pow(r, k - 2);
+        B[4 * i + 2] = lap_k_phs_coef(k, 1) * pow(r, k - 2);
+        B[4 * i + 3] = lap_k_phs_coef(k, 2) * pow(r, k - 4);
         /* right-hand sides Delta^kk, Delta, Delta^2 of r^K at the centre */
-        B2[3 * i] = kk > 0 ? cL * pow(r, K - 2 * kk) : 0.0;
-        B2[3 * i + 1] = lap_k_phs_coef(K, 1) * pow(r, K - 2);
-        B2[3 * i + 2] = lap_k_phs_coef(K, 2) * pow(r, K - 4);
+        if (gauss) {
+            B2[i] = L_ga(r, geps, kk);
+        } else {
+            B2[i] = kk > 0 ? cL * pow(r, K - 2 * kk) : 0.0;
+        }
     }
     int off = n;
     for (int p = 0; p <= d; p++) {
@@ -280,13 +290,13 @@ static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, con
                 A2[(size_t)(off + i) * m + j] = a;
                 A2[(size_t)j * m + off + i] = a;
             }
-            B[2 * (off + i)] = (p == 1 && i == 0) ? 1.0 : 0.0;
-            B[2 * (off + i) + 1] = (p == 1 && i == 1) ? 1.0 : 0.0;
+            B[4 * (off + i)] = (p == 1 && i == 0) ? 1.0 : 0.0;
+            B[4 * (off + i) + 1] = (p == 1 && i == 1) ? 1.0 : 0.0;
             /* Delta, Delta^2 of x^(p-i) y^i at the origin: 2 for x^2, y^2; 24, 8, 24 for
              * x^4, x^2 y^2, y^4; Delta^kk (kk >= 3) of degree < 2kk vanishes there */
-            B2[3 * (off + i)] = 0.0;
-            B2[3 * (off + i) + 1] = (p == 2 && (i == 0 || i == 2)) ? 2.0 : 0.0;
-            B2[3 * (off + i) + 2] = p == 4 ? ((i == 0 || i == 4) ? 24.0 : (i == 2 ? 8.0 : 0.0)) : 0.0;
+            B[4 * (off + i) + 2] = (p == 2 && (i == 0 || i == 2)) ? 2.0 : 0.0;
+            B[4 * (off + i) + 3] = p == 4 ? ((i == 0 || i == 4) ? 24.0 : (i == 2 ? 8.0 : 0.0)) : 0.0;
+            B2[off + i] = 0.0;
         }
         off += p + 1;
     }
@@ -295,17 +305,31 @@ static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, con
             A[(size_t)i * m + j] = 0.0;
             A2[(size_t)i * m + j] = 0.0;
         }
-    int info1 = orc_dgesv(m, 2, A, ipiv, B);
+    int info1 = orc_dgesv(m, 4, A, ipiv, B);
     *info2 = 0;
-    if (kk > 0)
-        *info2 = orc_dgesv(m, 3, A2, ipiv, B2);
+    if (gauss) {
+        /* Gaussian + constant: rows/columns 0..n, the constant row's right-hand sides 0 */
+        const int m2 = n + 1;
+        double *G = malloc(sizeof(double) * (size_t)m2 * m2);
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < n; j++)
+                G[(size_t)i * m2 + j] = A2[(size_t)i * m + j];
+            G[(size_t)i * m2 + n] = 1.0;
+            G[(size_t)n * m2 + i] = 1.0;
+        }
+        G[(size_t)n * m2 + n] = 0.0;
+        B2[n] = 0.0;
+        *info2 = orc_dgesv(m2, 1, G, ipiv, B2);
+        free(G);
+    } else if (kk > 0)
+        *info2 = orc_dgesv(m, 1, A2, ipiv, B2);
     double s2k = pow(s, 2 * kk);
     for (int j = 0; j < n; j++) {
-        wx[j] = B[2 * j] / s;
-        wy[j] = B[2 * j + 1] / s;
-        wl[j] = kk > 0 ? B2[3 * j] / s2k : 0.0;
-        if (wl1) wl1[j] = kk > 0 ? B2[3 * j + 1] / (s * s) : 0.0;
-        if (wl2) wl2[j] = kk > 0 ? B2[3 * j + 2] / (s * s * s * s) : 0.0;
+        wx[j] = B[4 * j] / s;
+        wy[j] = B[4 * j + 1] / s;
+        wl[j] = kk > 0 ? B2[j] / s2k : 0.0;
+        if (wl1) wl1[j] = B[4 * j + 2] / (s * s);
+        if (wl2) wl2[j] = B[4 * j + 3] / (s * s * s * s);
     }
     free(xs); free(A); free(B); free(A2); free(B2); free(ipiv);
     return info1;

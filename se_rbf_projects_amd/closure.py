@@ -3,18 +3,18 @@ training with the library's BF16 matrix-core forward/backward and Adam kernels, 
 a-posteriori comparison with the constant-coefficient method.
 
 The closure multiplies the hyperviscosity coefficient point by point, c = hv_coef * s, with
-s = softplus(MLP(features)) (the features and the parameter layout are documented in
-include/serbf.h).  Training data, per snapshot time t_k of a fine FP64 run:
+s = min(softplus(MLP(features)), s_max) (the features and the parameter layout are documented
+in include/serbf.h).  Training data, per snapshot time t_k of a fine FP64 run:
 
     q_f(t_k), q_f(t_k + dt_c)        fine run restricted to the coarse nodes (each icosahedral
                                      level is a prefix of the next: restriction is injection)
-    q_c = one coarse step from q_f(t_k) with the constant coefficient, hyperviscosity split
-          (applied once per step as dt c Delta^k q(t_k)), so that a coefficient c s gives
-          exactly q_c + dt_c hv_coef (s - 1) Delta^k q(t_k)
+    q_c = one coarse RK4 step from q_f(t_k) with the constant coefficient; to first order a
+          coefficient c s gives q_c + dt_c hv_coef (s - 1) Delta^k q(t_k)
     A = dt_c hv_coef Delta^k q(t_k),   R = q_f(t_k + dt_c) - q_c
 
 and the loss of a point is ((s - 1) A - R)^2: s = 1 (the constant method) scores mean(R^2),
-which is normalised to 1.
+which is normalised to 1.  s_max (default 16) keeps hv_coef s inside the RK4 stability region
+of the Gaussian Delta^4 operator at the configured coefficient.
 
     python -m se_rbf_projects_amd.closure train [--fine-level 8 --coarse-level 6 --levels 30]
 """
@@ -109,7 +109,6 @@ def make_dataset(fine_level=8, coarse_level=6, Nv=30, n=55, deg=4, nv=7, snapsho
     dt_f = dt_c / ratio
     fine = _engine(xyz_all[perm_f], cfg_f)
     coarse = _engine(xyz_c, cfg_c)
-    coarse.set_hv_split(True)
     q0 = np.stack([E.initial_condition(k, cfg_f, xyz_all[perm_f], seed=seed) for k in TRACERS], axis=1)
     fine.set_state(q0)
     log(f"closure data: fine {Nf}x{Nv} dt {dt_f:.1f}s, coarse {Nc}x{Nv} dt {dt_c:.1f}s, setup {time.time() - t0:.1f}s")
@@ -157,7 +156,10 @@ def optimal_constant(A, R):
     return s, float(np.mean(((s - 1.0) * A - R) ** 2))
 
 
-def train(feat, A, R, steps=4000, batch=8192, lr=2e-3, holdout=0.1, seed=0, log_every=500):
+S_MAX = 16.0
+
+
+def train(feat, A, R, steps=4000, batch=8192, lr=2e-3, holdout=0.1, seed=0, log_every=500, s_max=S_MAX):
     """Train on a shuffled split; returns (params, norm, report)."""
     m = len(A) // 16 * 16
     rng = np.random.default_rng(seed)
@@ -170,6 +172,7 @@ def train(feat, A, R, steps=4000, batch=8192, lr=2e-3, holdout=0.1, seed=0, log_
     norm = feature_norm(feat[:ntr])
     params = init_params(seed)
     cl = E.Closure(0)
+    cl.set_smax(s_max)
     cl.set_params(params, norm)
     cl.set_data(feat, A, R)
     _, loss0_tr = cl.grad(0, ntr)
@@ -186,8 +189,8 @@ def train(feat, A, R, steps=4000, batch=8192, lr=2e-3, holdout=0.1, seed=0, log_
     _, loss_ho = cl.grad(ntr, nho)
     s_opt, loss_opt = optimal_constant(A[ntr:], R[ntr:])
     o = cl.eval(ntr, nho)
-    sv = np.log1p(np.exp(np.minimum(o, 30.0)))
-    report = dict(samples=int(m), train=int(ntr), holdout=int(nho), steps=steps, batch=batch, lr=lr,
+    sv = np.minimum(np.log1p(np.exp(np.minimum(o, 30.0))), s_max)
+    report = dict(samples=int(m), train=int(ntr), holdout=int(nho), steps=steps, batch=batch, lr=lr, s_max=s_max,
                   train_seconds=secs, loss_const_train=loss0_tr, loss_const_holdout=loss0_ho,
                   loss_trained_train=loss_tr, loss_trained_holdout=loss_ho,
                   best_constant_multiplier=s_opt, loss_best_constant_holdout=loss_opt,
@@ -208,7 +211,8 @@ def load(path=PARAMS_PATH):
         return z["params"].astype(np.float32), z["norm"].astype(np.float32), report
 
 
-def a_posteriori(params, norm, level=6, Nv=30, n=55, deg=4, nv=7, seed=2024, every=1, precision=F64):
+def a_posteriori(params, norm, s_max=S_MAX, level=6, Nv=30, n=55, deg=4, nv=7, seed=2024, every=8,
+                 precision=F64):
     """Full-period runs of the deformational flow at one resolution with the constant coefficient
     and with the closure; the exact solution at t = T is q(0).  Returns relative l2/linf errors."""
     xyz = E.icos_nodes(level)
@@ -219,14 +223,13 @@ def a_posteriori(params, norm, level=6, Nv=30, n=55, deg=4, nv=7, seed=2024, eve
     nsteps = int(math.ceil(cfg.T / courant_dt(cfg, N)))
     dt = cfg.T / nsteps
     eng = _engine(xyz, cfg)
-    eng.set_hv_split(True)
     q0 = np.stack([E.initial_condition(k, cfg, xyz, seed=seed) for k in TRACERS], axis=1)
     out = {}
     cl = E.Closure(0)
     cl.set_params(params, norm)
     for name in ("constant", "closure"):
         eng.set_state(q0)
-        eng.attach_closure(cl if name == "closure" else None, every=every)
+        eng.attach_closure(cl if name == "closure" else None, every=every, s_max=s_max)
         t = time.time()
         eng.run(precision, nsteps, dt)
         secs = time.time() - t
@@ -263,7 +266,7 @@ def main(argv=None):
     rep["data"] = meta
     log(json.dumps({k: v for k, v in rep.items() if k != "data"}))
     if not a.no_posteriori:
-        rep["a_posteriori"] = a_posteriori(params, norm, level=a.coarse_level, Nv=a.levels)
+        rep["a_posteriori"] = a_posteriori(params, norm, s_max=rep["s_max"], level=a.coarse_level, Nv=a.levels)
         log(json.dumps(rep["a_posteriori"]))
     save(params, norm, rep, a.out)
     if a.report:

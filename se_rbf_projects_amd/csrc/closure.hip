@@ -30,6 +30,7 @@ struct serbf_closure {
     int loss_cap = 0;
     float *out = nullptr;
     long out_cap = 0;
+    double s_max = 16.0;   // clamp of the multiplier in the training loss
 };
 
 namespace serbf {
@@ -232,9 +233,9 @@ __global__ void __launch_bounds__(256) features_kernel(FeatArgs<T> a, float *fea
 }
 
 // closure coefficients: gamma[(p*ntr + tr)*S + v] = hv_coef * s, s = softplus(MLP(x)), with
-// s = 1 where s is not finite or above s_max; levels >= Nv get 0
+// s clamped to s_max, s = 1 where it is not finite; levels >= Nv get 0
 template <typename T, int S>
-__global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, float hv_coef, float s_max)
+__global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, T hv_coef, float s_max)
 {
     constexpr int CPW = 64 / S;
     __shared__ alignas(16) unsigned short xs[4][64 * 16];
@@ -270,8 +271,9 @@ __global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T
     }
     wave_sync();
     float sv = softplus_f(os[w][lane]);
-    if (!(sv >= 0.f && sv <= s_max)) sv = 1.f;
-    if (ok) gamma[((size_t)p * a.ntr + tr) * S + v] = v < a.s.Nv ? (T)(hv_coef * sv) : T(0);
+    if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
+    if (sv > s_max) sv = s_max;
+    if (ok) gamma[((size_t)p * a.ntr + tr) * S + v] = v < a.s.Nv ? hv_coef * (T)sv : T(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -279,7 +281,8 @@ __global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T
 
 // one wavefront per strided set of 16-sample tiles; writes its gradient record to partials
 __global__ void __launch_bounds__(256) train_kernel(NetView nv, const float *feat, const float *A, const float *R,
-                                                    long first, int ntiles, float inv_count, float *partials)
+                                                    long first, int ntiles, float inv_count, float s_max,
+                                                    float *partials)
 {
     __shared__ alignas(16) unsigned short w2t[kH * kH];
     __shared__ alignas(16) unsigned short h1s[4][kH * 16];
@@ -322,9 +325,12 @@ __global__ void __launch_bounds__(256) train_kernel(NetView nv, const float *fea
         for (int i = 0; i < 4; i++) {
             const long si = s0 + 4 * g + i;
             const float av = A[si], rv = R[si];
-            const float e = (softplus_f(o[i]) - 1.f) * av - rv;
+            // s = min(softplus(o), s_max): no gradient through the clamp
+            const float sp = softplus_f(o[i]);
+            const bool clamped = sp > s_max;
+            const float e = ((clamped ? s_max : sp) - 1.f) * av - rv;
             if (c == 0) loss = fmaf(e, e, loss);
-            go[i] = 2.f * e * av * sigmoid_f(o[i]) * inv_count;
+            go[i] = clamped ? 0.f : 2.f * e * av * sigmoid_f(o[i]) * inv_count;
         }
         if (c == 0) db3 += (go[0] + go[1]) + (go[2] + go[3]);
         float dz2[4][4];
@@ -513,7 +519,7 @@ int run_grad(serbf_closure *c, long first, long count, bool update, float lr, in
     const int ntiles = (int)(count / 16);
     const int blocks = launch_grid_waves(ntiles);
     hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(256), 0, c->stream, view(c), c->feat, c->A, c->R, first,
-                       ntiles, 1.f / (float)count, c->partials);
+                       ntiles, 1.f / (float)count, (float)c->s_max, c->partials);
     SERBF_HIP(hipGetLastError());
     float bc1 = 1.f, bc2 = 1.f;
     if (update) {
@@ -579,7 +585,8 @@ int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma)
     const long waves = e->S == 16 ? (ncol + 3) / 4 : ncol;
     const dim3 grid((unsigned)((waves + 3) / 4));
     const NetView nv = view(e->closure);
-    const float hv = (float)e->cfg.hv_coef, smax = (float)e->closure_smax;
+    const T hv = (T)e->cfg.hv_coef;
+    const float smax = (float)e->closure_smax;
     if (e->S == 16) hipLaunchKernelGGL((apply_kernel<T, 16>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax);
     else hipLaunchKernelGGL((apply_kernel<T, 64>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax);
     SERBF_HIP(hipGetLastError());
@@ -665,6 +672,13 @@ int serbf_closure_set_params(serbf_closure *c, const float *params, const float 
     SERBF_HIP(hipStreamSynchronize(c->stream));
     c->step = 0;
     c->have_params = true;
+    return SERBF_OK;
+}
+
+int serbf_closure_set_smax(serbf_closure *c, double s_max)
+{
+    SERBF_CHECK(c && s_max > 1.0, SERBF_ERR_ARG, "closure_set_smax: s_max must exceed 1");
+    c->s_max = s_max;
     return SERBF_OK;
 }
 

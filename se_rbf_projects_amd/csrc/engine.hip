@@ -221,8 +221,10 @@ int serbf_configure(serbf_engine *e, const serbf_config *cfg)
         SERBF_CHECK(c.poly_deg >= 0 && c.poly_deg <= 6, SERBF_ERR_UNSUPPORTED, "configure: poly_deg");
         SERBF_CHECK(c.hv_order == 0 || c.poly_deg < 2 * c.hv_order, SERBF_ERR_UNSUPPORTED,
                     "configure: poly_deg must be below 2*hv_order");
-        SERBF_CHECK(c.hv_order == 0 || (c.hv_phs_order >= 2 * c.hv_order + 1 && (c.hv_phs_order & 1)),
-                    SERBF_ERR_ARG, "configure: hv_phs_order must be odd and >= 2k+1");
+        SERBF_CHECK(c.hv_order == 0 || (c.hv_phs_order == 0 && c.hv_eps > 0.0) ||
+                        (c.hv_phs_order >= 2 * c.hv_order + 1 && (c.hv_phs_order & 1)),
+                    SERBF_ERR_ARG,
+                    "configure: hv_phs_order must be 0 (Gaussian, hv_eps > 0) or odd and >= 2k+1");
         SERBF_CHECK(c.phs_order & 1, SERBF_ERR_ARG, "configure: phs_order must be odd");
     } else {
         SERBF_CHECK(c.hv_order >= 0 && c.hv_order < SERBF_REF_MAX_HV_ORDER, SERBF_ERR_ARG, "configure: hv_order");

@@ -37,7 +37,7 @@ struct WArgs {
     const double *x, *y, *z;
     const int *idx;
     double *Wx, *Wy, *WL;
-    double *WL1, *WL2;   // config mode: Delta and Delta^2 weights from the Delta^k system (may be null)
+    double *WL1, *WL2;   // config mode: Delta and Delta^2 weights from the gradient system (may be null)
     int *status;
     int *counter;
 };
@@ -77,25 +77,29 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // LDS layout of one wavefront's system (doubles):
-//   A   [MAXM][LD]  augmented matrix [A | B] (LD >= MAXM + 3, odd)
+//   A   [MAXM][LD]  augmented matrix [A | B] (LD >= MAXM + 4, odd)
 //   X   [3][MAXN]   stencil coordinates in the solve frame
-//   B0  [MAXM][3]   right-hand sides (kept for the residual)
-//   XS  [MAXM][3]   solution
+//   B0  [MAXM][4]   right-hand sides (kept for the residual)
+//   XS  [MAXM][4]   solution
 //   piv [MAXM]      pivot rows (int)
+// right-hand-side columns per system (gradient system: d/dx', d/dy', Delta, Delta^2)
+constexpr int kRhs = 4;
+
 template <int MAXM>
 struct Lds {
-    static constexpr int LD = ((MAXM + 3) | 1);
+    static constexpr int LD = ((MAXM + 4) | 1);
     static constexpr int MAXN = MAXM;
-    static constexpr int doubles = MAXM * LD + 3 * MAXN + 2 * MAXM * 3;
+    static constexpr int doubles = MAXM * LD + 3 * MAXN + 2 * MAXM * kRhs;
     static constexpr int bytes = doubles * 8 + MAXM * 4;
 };
 
 // Matrix entry A_ij of the current system (rows/cols < m); kind 0: gradient PHS system,
-// kind 1: Gaussian (reference) or PHS r^K (config) Delta^k system.
+// kind 1: Delta^k system, Gaussian + constant (reference mode, or config mode with
+// hv_phs_order 0; a.eps is the shape parameter in the solve frame) or PHS r^K + monomials.
 __device__ __forceinline__ double entry(int i, int j, int n, int m, int kind, const WArgs &a, const double *X,
                                         int MAXN)
 {
-    const bool gauss = kind == 1 && a.ref_mode;
+    const bool gauss = kind == 1 && (a.ref_mode || a.hvphs == 0);
     if (i < n && j < n) {
         const double dx = X[j] - X[i], dy = X[MAXN + j] - X[MAXN + i], dz = X[2 * MAXN + j] - X[2 * MAXN + i];
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -195,19 +199,20 @@ __device__ int factor_solve(double *A, double *X, double *B0, double *XS, int *p
         wave_sync();
     }
     for (int i = lane; i < m; i += 64)
-        for (int c = 0; c < nrhs; c++) XS[i * 3 + c] = A[i * LD + m + c];
+        for (int c = 0; c < nrhs; c++) XS[i * kRhs + c] = A[i * LD + m + c];
     wave_sync();
     // ---- one refinement step: r = b - A x with A re-evaluated, solve with the factors ----
-    double res[R][3];
+    double res[R][kRhs];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int i = lane + 64 * r;
-        for (int c = 0; c < 3; c++) res[r][c] = 0.0;
+        for (int c = 0; c < kRhs; c++) res[r][c] = 0.0;
         if (i < m) {
-            double acc[3] = {B0[i * 3], B0[i * 3 + 1], B0[i * 3 + 2]};
+            double acc[kRhs];
+            for (int c = 0; c < kRhs; c++) acc[c] = B0[i * kRhs + c];
             for (int j = 0; j < m; j++) {
                 const double aij = entry(i, j, n, m, kind, a, X, MAXN);
-                for (int c = 0; c < nrhs; c++) acc[c] -= aij * XS[j * 3 + c];
+                for (int c = 0; c < nrhs; c++) acc[c] -= aij * XS[j * kRhs + c];
             }
             for (int c = 0; c < nrhs; c++) res[r][c] = acc[c];
         }
@@ -255,7 +260,7 @@ __device__ int factor_solve(double *A, double *X, double *B0, double *XS, int *p
         wave_sync();
     }
     for (int i = lane; i < m; i += 64)
-        for (int c = 0; c < nrhs; c++) XS[i * 3 + c] += A[i * LD + m + c];
+        for (int c = 0; c < nrhs; c++) XS[i * kRhs + c] += A[i * LD + m + c];
     wave_sync();
     return 0;
 }
@@ -267,45 +272,46 @@ __device__ void assemble(double *A, double *X, double *B0, int n, int m, int nrh
     constexpr int LD = Lds<MAXM>::LD, MAXN = Lds<MAXM>::MAXN;
     for (int i = lane; i < m; i += 64) {
         for (int j = 0; j < m; j++) A[i * LD + j] = entry(i, j, n, m, kind, a, X, MAXN);
-        double b[3] = {0.0, 0.0, 0.0};
+        double b[kRhs] = {0.0, 0.0, 0.0, 0.0};
         if (i < n) {
             // derivative at the centre (stencil node 0) of the basis function centred at node i
             const double dx = X[0] - X[i], dy = X[MAXN] - X[MAXN + i], dz = X[2 * MAXN] - X[2 * MAXN + i];
             const double r = sqrt(dx * dx + dy * dy + dz * dz);
             if (kind == 0) {
+                // d/dx', d/dy', and (config mode) Delta, Delta^2 of r^k at the centre:
+                // Delta^j r^k = prod_{i<j} (k - 2i)^2 r^(k - 2j) (2D radial Laplacian)
                 const int k = a.phs;
+                const double K = (double)k;
                 b[0] = k * dx * phs_pow(r, k - 2);
                 b[1] = k * dy * phs_pow(r, k - 2);
-            } else if (a.ref_mode) {
+                b[2] = K * K * phs_pow(r, k - 2);
+                b[3] = K * K * (K - 2.0) * (K - 2.0) * phs_pow(r, k - 4);
+            } else if (a.ref_mode || a.hvphs == 0) {
                 b[0] = lap_gauss(r, a.eps, a.hv);
             } else {
-                // Delta^j r^K = prod_{i<j} (K - 2i)^2 r^(K - 2j) (2D radial Laplacian), j = k, 1, 2
                 const double K = (double)a.hvphs;
                 double c = 1.0;
                 for (int jj = 0; jj < a.hv; jj++) c *= (K - 2 * jj) * (K - 2 * jj);
                 b[0] = c * phs_pow(r, a.hvphs - 2 * a.hv);
-                b[1] = K * K * phs_pow(r, a.hvphs - 2);
-                b[2] = K * K * (K - 2.0) * (K - 2.0) * phs_pow(r, a.hvphs - 4);
             }
         } else if (kind == 0) {
-            // d/dx' of x' and d/dy' of y' at the origin of the tangent plane
-            b[0] = (i - n == 1) ? 1.0 : 0.0;
-            b[1] = (i - n == 2) ? 1.0 : 0.0;
-        } else if (!a.ref_mode) {
-            // Delta and Delta^2 of the monomial x^(p-q) y^q at the origin (Delta^k, k >= 4, of a
-            // monomial of degree < 2k vanishes there)
+            // d/dx' of x' and d/dy' of y' at the origin of the tangent plane; Delta and Delta^2 of
+            // the monomial x^(p-q) y^q there (2 for x^2, y^2; 24, 8, 24 for x^4, x^2 y^2, y^4).
+            // Delta^k (k >= 3) of a monomial of degree < 2k vanishes there.
             const int t = i - n;
             int p = 0;
             while ((p + 1) * (p + 2) / 2 <= t) p++;
             const int q = t - p * (p + 1) / 2;
-            b[1] = (p == 2 && (q == 0 || q == 2)) ? 2.0 : 0.0;
-            b[2] = p == 4 ? ((q == 0 || q == 4) ? 24.0 : (q == 2 ? 8.0 : 0.0)) : 0.0;
+            b[0] = (t == 1) ? 1.0 : 0.0;
+            b[1] = (t == 2) ? 1.0 : 0.0;
+            b[2] = (p == 2 && (q == 0 || q == 2)) ? 2.0 : 0.0;
+            b[3] = p == 4 ? ((q == 0 || q == 4) ? 24.0 : (q == 2 ? 8.0 : 0.0)) : 0.0;
         }
         for (int c = 0; c < nrhs; c++) {
             A[i * LD + m + c] = b[c];
-            B0[i * 3 + c] = b[c];
+            B0[i * kRhs + c] = b[c];
         }
-        for (int c = nrhs; c < 3; c++) B0[i * 3 + c] = 0.0;
+        for (int c = nrhs; c < kRhs; c++) B0[i * kRhs + c] = 0.0;
     }
     wave_sync();
 }
@@ -318,13 +324,14 @@ __global__ void __launch_bounds__(64) weights_kernel(WArgs a)
     double *A = smem;
     double *X = A + MAXM * LD;
     double *B0 = X + 3 * MAXN;
-    double *XS = B0 + MAXM * 3;
-    int *piv = (int *)(XS + MAXM * 3);
+    double *XS = B0 + MAXM * kRhs;
+    int *piv = (int *)(XS + MAXM * kRhs);
     const int lane = threadIdx.x;
     const int n = a.n;
     const int M = a.ref_mode ? 21 : (a.deg + 1) * (a.deg + 2) / 2;
     const int m1 = n + M;
-    const int m2 = a.ref_mode ? n + 1 : n + M;
+    const bool gauss_hv = a.ref_mode || a.hvphs == 0;
+    const int m2 = gauss_hv ? n + 1 : n + M;
     for (;;) {
         int node = 0;
         if (lane == 0) node = atomicAdd(a.counter, 1);
@@ -361,29 +368,32 @@ __global__ void __launch_bounds__(64) weights_kernel(WArgs a)
             wave_sync();
         }
         int st = 0;
-        // gradient system
-        assemble<MAXM>(A, X, B0, n, m1, 2, 0, a, lane);
-        if (factor_solve<MAXM>(A, X, B0, XS, piv, n, m1, 2, 0, a, lane)) st = 1;
+        // gradient system (config mode with closure weights: also Delta and Delta^2)
+        const int ng = a.WL1 ? 4 : 2;
+        assemble<MAXM>(A, X, B0, n, m1, ng, 0, a, lane);
+        if (factor_solve<MAXM>(A, X, B0, XS, piv, n, m1, ng, 0, a, lane)) st = 1;
         for (int s = lane; s < n; s += 64) {
-            a.Wx[(size_t)node * n + s] = st ? 0.0 : XS[s * 3] / scale;
-            a.Wy[(size_t)node * n + s] = st ? 0.0 : XS[s * 3 + 1] / scale;
+            a.Wx[(size_t)node * n + s] = st ? 0.0 : XS[s * kRhs] / scale;
+            a.Wy[(size_t)node * n + s] = st ? 0.0 : XS[s * kRhs + 1] / scale;
+        }
+        if (a.WL1) {
+            const double s2 = scale * scale, s4 = s2 * s2;
+            for (int s = lane; s < n; s += 64) {
+                a.WL1[(size_t)node * n + s] = st ? 0.0 : XS[s * kRhs + 2] / s2;
+                a.WL2[(size_t)node * n + s] = st ? 0.0 : XS[s * kRhs + 3] / s4;
+            }
         }
         wave_sync();
         // Delta^k system
         if (a.hv > 0) {
-            const int nr = a.WL1 ? 3 : 1;
-            assemble<MAXM>(A, X, B0, n, m2, nr, 1, a, lane);
-            int bad = factor_solve<MAXM>(A, X, B0, XS, piv, n, m2, nr, 1, a, lane);
+            // the Gaussian shape parameter of the unit sphere, in the scaled solve frame
+            WArgs ag = a;
+            ag.eps = a.eps * scale;
+            assemble<MAXM>(A, X, B0, n, m2, 1, 1, ag, lane);
+            int bad = factor_solve<MAXM>(A, X, B0, XS, piv, n, m2, 1, 1, ag, lane);
             if (bad && !st) st = 2;
             const double s2k = pow(scale, 2.0 * a.hv);
-            for (int s = lane; s < n; s += 64) a.WL[(size_t)node * n + s] = bad ? 0.0 : XS[s * 3] / s2k;
-            if (a.WL1) {
-                const double s2 = scale * scale, s4 = s2 * s2;
-                for (int s = lane; s < n; s += 64) {
-                    a.WL1[(size_t)node * n + s] = bad ? 0.0 : XS[s * 3 + 1] / s2;
-                    a.WL2[(size_t)node * n + s] = bad ? 0.0 : XS[s * 3 + 2] / s4;
-                }
-            }
+            for (int s = lane; s < n; s += 64) a.WL[(size_t)node * n + s] = bad ? 0.0 : XS[s * kRhs] / s2k;
         } else {
             for (int s = lane; s < n; s += 64) a.WL[(size_t)node * n + s] = 0.0;
         }
@@ -428,7 +438,7 @@ int weights_run(serbf_engine *e)
     a.x = e->d_x; a.y = e->d_y; a.z = e->d_z;
     a.idx = e->d_idx;
     a.Wx = e->d_Wx; a.Wy = e->d_Wy; a.WL = e->d_WL;
-    const bool lap = c.mode == SERBF_MODE_CONFIG && c.hv_order >= 3;
+    const bool lap = c.mode == SERBF_MODE_CONFIG && c.poly_deg >= 2;
     a.WL1 = lap ? e->d_WL1 : nullptr;
     a.WL2 = lap ? e->d_WL2 : nullptr;
     a.status = e->d_status;

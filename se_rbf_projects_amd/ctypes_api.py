@@ -66,28 +66,52 @@ def hex_spacing(N):
     return float(np.sqrt(8 * np.pi / (np.sqrt(3) * N)))
 
 
-def config_A(N=2562):
+# hyperviscosity coefficient gamma_c of hv_coef = -gamma_c h^8 u0 / (R h) (see hv_coefficient)
+HV_GAMMA = 2.0 ** -14
+
+
+def reference_eps(N):
+    """Gaussian shape parameter of the unit sphere, eps = 0.044 sqrt(N) - 0.14 (wrappers.c:209-211)."""
+    return 0.044 * float(np.sqrt(N)) - 0.14
+
+
+def hv_coefficient(N, u0, gamma_c=None):
+    """hv_coef = -gamma_c h^8 u0 / (R h): the dimensionless -gamma_c h^8 of the configs applied as
+    a rate, in units of the cell-crossing frequency u0 / (R h)."""
+    h = hex_spacing(N)
+    g = HV_GAMMA if gamma_c is None else gamma_c
+    return -g * h ** 8 * u0 / (EARTH_R * h)
+
+
+def _hv_fields(N, hv):
+    """hv = "gauss": Gaussian + constant Delta^4 with the reference's eps; "phs": PHS r^9 + the
+    gradient monomials."""
+    if hv == "gauss":
+        return dict(hv_phs_order=0, hv_eps=reference_eps(N))
+    if hv == "phs":
+        return dict(hv_phs_order=9, hv_eps=0.0)
+    raise ValueError(hv)
+
+
+def config_A(N=2562, hv="gauss", gamma_c=None):
     """Plain method, CPU-runnable: ICOS level 4, Nv = 10 to 12 km, n = 31, PHS r^7 +
     degree-3 tangent monomials, vertical n = 5, Delta^4 with gamma = -2^-14 h^8,
     tilted (45 deg) solid-body rotation with u0 = 2 pi R / 12 days."""
-    h = hex_spacing(N)
     u0 = 2 * np.pi * EARTH_R / (12 * DAY)
     return make_config(Nv=10, htop=12000.0, mode=MODE_CONFIG, n=31, phs_order=7, poly_deg=3, hv_order=4,
-                       hv_phs_order=9, nv=5, v_phs_order=5, v_poly_deg=3,
-                       hv_coef=-(2.0 ** -14) * h ** 8 * u0 / (EARTH_R * h), flow=FLOW_SOLID_BODY, u0=u0,
-                       alpha=np.pi / 4, T=12 * DAY)
+                       nv=5, v_phs_order=5, v_poly_deg=3, hv_coef=hv_coefficient(N, u0, gamma_c),
+                       flow=FLOW_SOLID_BODY, u0=u0, alpha=np.pi / 4, T=12 * DAY, **_hv_fields(N, hv))
 
 
-def config_deform(N, Nv=60, n=55, deg=4, nv=7):
+def config_deform(N, Nv=60, n=55, deg=4, nv=7, hv="gauss", gamma_c=None):
     """Deformational flow of the at-scale configs: solid-body rotation 2 pi R / 12 days
     plus a reversing swirl of amplitude 10 R / T and w = 0.15 sin(pi z/ztop) cos(pi t/T)."""
-    h = hex_spacing(N)
     T = 12 * DAY
     u0 = 2 * np.pi * EARTH_R / T
     return make_config(Nv=Nv, htop=12000.0, mode=MODE_CONFIG, n=n, phs_order=7, poly_deg=deg, hv_order=4,
-                       hv_phs_order=9, nv=nv, v_phs_order=5, v_poly_deg=nv - 2,
-                       hv_coef=-(2.0 ** -14) * h ** 8 * u0 / (EARTH_R * h), flow=FLOW_DEFORM, u0=u0,
-                       alpha=0.0, T=T, k_def=10 * EARTH_R / T, w0=0.15)
+                       nv=nv, v_phs_order=5, v_poly_deg=nv - 2, hv_coef=hv_coefficient(N, u0, gamma_c),
+                       flow=FLOW_DEFORM, u0=u0, alpha=0.0, T=T, k_def=10 * EARTH_R / T, w0=0.15,
+                       **_hv_fields(N, hv))
 
 
 def courant_dt(cfg, N, courant=0.5):

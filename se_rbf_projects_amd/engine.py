@@ -74,6 +74,7 @@ def lib():
             "serbf_closure_destroy": ([vp], None),
             "serbf_closure_set_params": ([vp, vp, vp], i),
             "serbf_closure_get_params": ([vp, vp], i),
+            "serbf_closure_set_smax": ([vp, d], i),
             "serbf_closure_set_data": ([vp, C.c_long, vp, vp, vp], i),
             "serbf_closure_grad": ([vp, C.c_long, C.c_long, vp], i),
             "serbf_closure_train": ([vp, C.c_long, C.c_long, i, i, d, vp], i),
@@ -159,6 +160,9 @@ class Closure:
         nm = np.ascontiguousarray(norm, np.float32)
         assert p.size == self.NPARAMS and nm.size == 32
         _check(lib().serbf_closure_set_params(self._c, _p(p), _p(nm)), "closure_set_params")
+
+    def set_smax(self, s_max):
+        _check(lib().serbf_closure_set_smax(self._c, float(s_max)), "closure_set_smax")
 
     def params(self):
         p = np.empty(self.NPARAMS, np.float32)

@@ -34,7 +34,7 @@ def lib():
         _lib.orc_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         _lib.orc_mlp_forward.restype = C.c_double
         _lib.orc_mlp_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
-                                      C.c_int, C.c_void_p]
+                                      C.c_int, C.c_double, C.c_void_p]
         _lib.orc_mlp_grad.restype = C.c_double
         _lib.orc_adam.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                                   C.c_double, C.c_double, C.c_double]
@@ -206,7 +206,7 @@ def mlp_forward(params, norm, feat, emulate=True):
                                            int(emulate), None, None) for i in range(len(f))])
 
 
-def mlp_grad(params, norm, feat, A, R, emulate=True):
+def mlp_grad(params, norm, feat, A, R, emulate=True, s_max=16.0):
     p = _c(params, np.float32)
     nm = _c(norm, np.float32)
     f = _c(feat, np.float32)
@@ -214,7 +214,7 @@ def mlp_grad(params, norm, feat, A, R, emulate=True):
     R = _c(R, np.float32)
     g = np.zeros(NPARAMS)
     loss = lib().orc_mlp_grad(as_ptr(p, C.c_float), as_ptr(nm, C.c_float), len(A), as_ptr(f, C.c_float),
-                              as_ptr(A, C.c_float), as_ptr(R, C.c_float), int(emulate), as_ptr(g))
+                              as_ptr(A, C.c_float), as_ptr(R, C.c_float), int(emulate), float(s_max), as_ptr(g))
     return g, loss
 
 

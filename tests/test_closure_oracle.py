@@ -69,6 +69,18 @@ def test_constant_method_loss_and_output_gradient_closed_form():
         assert g[CL.O_B3] == pytest.approx(-2 * sig * np.mean(A.astype(np.float64) * R), rel=1e-5)
 
 
+def test_clamped_multiplier_has_no_gradient():
+    """Above s_max the multiplier is clamped: the loss uses s_max and no gradient flows."""
+    params, norm, feat, A, R = _problem(9)
+    params[CL.O_W3:CL.O_B3] = 0.0
+    params[CL.O_B3] = 40.0                                   # softplus(40) = 40 > s_max
+    g, loss = O.mlp_grad(params, norm, feat, A, R, emulate=False, s_max=8.0)
+    assert np.all(g == 0.0)
+    assert loss == pytest.approx(np.mean((7.0 * A.astype(np.float64) - R) ** 2), rel=1e-12)
+    g2, _ = O.mlp_grad(params, norm, feat, A, R, emulate=False, s_max=64.0)
+    assert np.any(g2 != 0.0)
+
+
 def test_bf16_emulation_is_a_small_perturbation():
     params, norm, feat, A, R = _problem(7, m=64)
     g0, l0 = O.mlp_grad(params, norm, feat, A, R, emulate=False)

@@ -131,7 +131,7 @@ def _oracle_gamma(small, params, norm, t, q, s_max=8.0):
                                  small["dt"])
     o = O.mlp_forward(params, norm, feat, emulate=True)
     s = np.log1p(np.exp(np.minimum(o, 30.0)))
-    s = np.where(np.isfinite(s) & (s >= 0) & (s <= s_max), s, 1.0)
+    s = np.where(np.isfinite(s), np.minimum(s, s_max), 1.0)
     return cfg.hv_coef * s.reshape(q.shape), feat
 
 
@@ -151,13 +151,19 @@ def test_coefficients_match_oracle_and_fall_back(small):
     g32 = eng.closure_gamma(F32, t, small["dt"])
     rel32 = np.abs(g32 - ref) / abs(cfg.hv_coef)
     assert np.mean(rel32 < 1e-2) > 0.99 and rel32.max() < 1e-1
-    # an output far above s_max falls back to the constant coefficient everywhere
+    # an output far above s_max is clamped to s_max everywhere
     big = params.copy()
     big[CL.O_W3:CL.O_B3] = 0.0
     big[CL.O_B3] = 50.0
     cl.set_params(big, norm)
     gb = eng.closure_gamma(F64, t, small["dt"])
-    assert np.all(gb == cfg.hv_coef)
+    assert np.all(gb == 8.0 * cfg.hv_coef)
+    # a network that returns NaN falls back to the constant coefficient everywhere
+    bad = params.copy()
+    bad[CL.O_B3] = np.nan
+    cl.set_params(bad, norm)
+    gn = eng.closure_gamma(F64, t, small["dt"])
+    assert np.all(gn == cfg.hv_coef)
     eng.attach_closure(None)
     cl.close()
 

@@ -140,7 +140,7 @@ def test_laplacian_weights_reproduce_polynomials(golden, deg):
     Delta x^2 = Delta y^2 = 2, Delta^2 x^4 = Delta^2 y^4 = 24, Delta^2 x^2 y^2 = 8, others 0
     at the centre.  A wrong sign, power or scale in the right-hand sides fails here."""
     xyz = golden.icos(642)[:, :3]
-    cfg = config_A(642) if deg == 3 else config_deform(642, Nv=10, n=31, deg=4, nv=5)
+    cfg = config_A(642, hv="phs") if deg == 3 else config_deform(642, Nv=10, n=31, deg=4, nv=5, hv="phs")
     idx, _ = O.knn(xyz, cfg.n)
     Dx, Dy, L, L1, L2, st = O.weights_lap(cfg, xyz, idx)
     assert not st.any()
