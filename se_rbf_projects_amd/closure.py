@@ -6,15 +6,16 @@ The closure multiplies the hyperviscosity coefficient point by point, c = hv_coe
 s = min(softplus(MLP(features)), s_max) (the features and the parameter layout are documented
 in include/serbf.h).  Training data, per snapshot time t_k of a fine FP64 run:
 
-    q_f(t_k), q_f(t_k + dt_c)        fine run restricted to the coarse nodes (each icosahedral
+    q_f(t_k), q_f(t_k + M dt_c)      fine run restricted to the coarse nodes (each icosahedral
                                      level is a prefix of the next: restriction is injection)
-    q_c = one coarse RK4 step from q_f(t_k) with the constant coefficient; to first order a
-          coefficient c s gives q_c + dt_c hv_coef (s - 1) Delta^k q(t_k)
-    A = dt_c hv_coef Delta^k q(t_k),   R = q_f(t_k + dt_c) - q_c
+    q_1, q_hi = M coarse RK4 steps from q_f(t_k) with the coefficient hv_coef and s_hi hv_coef
+    A = (q_hi - q_1) / (s_hi - 1)    the response of the M-step solution to more damping
+    R = q_f(t_k + M dt_c) - q_1      the error the constant coefficient leaves
 
-and the loss of a point is ((s - 1) A - R)^2: s = 1 (the constant method) scores mean(R^2),
-which is normalised to 1.  s_max (default 16) keeps hv_coef s inside the RK4 stability region
-of the Gaussian Delta^4 operator at the configured coefficient.
+so that, linearised in a point's multiplier s, the coarse solution after M steps is
+q_1 + (s - 1) A, and the loss of a point is ((s - 1) A - R)^2: s = 1 (the constant method)
+scores mean(R^2), which is normalised to 1.  s_max = s_hi (default 16) keeps hv_coef s inside
+the RK4 stability region of the Gaussian Delta^4 operator at the configured coefficient.
 
     python -m se_rbf_projects_amd.closure train [--fine-level 8 --coarse-level 6 --levels 30]
 """
@@ -84,7 +85,7 @@ def _engine(xyz, cfg):
 
 
 def make_dataset(fine_level=8, coarse_level=6, Nv=30, n=55, deg=4, nv=7, snapshots=8, span=0.5, seed=2024,
-                 train_tracers=3):
+                 train_tracers=3, window=8, s_hi=16.0):
     """Training samples from a fine FP64 run and one-step coarse runs (see the module docstring).
     Returns feat [m, 16], A [m], R [m] (A and R divided by rms(R)), and metadata."""
     t0 = time.time()
@@ -101,7 +102,9 @@ def make_dataset(fine_level=8, coarse_level=6, Nv=30, n=55, deg=4, nv=7, snapsho
 
     cfg_f = config_deform(Nf, Nv=Nv, n=n, deg=deg, nv=nv)
     cfg_c = config_deform(Nc, Nv=Nv, n=n, deg=deg, nv=nv)
-    cfg_f.ntracers = cfg_c.ntracers = 4
+    cfg_hi = config_deform(Nc, Nv=Nv, n=n, deg=deg, nv=nv)
+    cfg_hi.hv_coef = s_hi * cfg_c.hv_coef
+    cfg_f.ntracers = cfg_c.ntracers = cfg_hi.ntracers = 4
     T = cfg_c.T
     ratio = 2 ** (fine_level - coarse_level)
     nsteps_c = int(math.ceil(T / courant_dt(cfg_c, Nc)))
@@ -109,28 +112,33 @@ def make_dataset(fine_level=8, coarse_level=6, Nv=30, n=55, deg=4, nv=7, snapsho
     dt_f = dt_c / ratio
     fine = _engine(xyz_all[perm_f], cfg_f)
     coarse = _engine(xyz_c, cfg_c)
+    coarse_hi = _engine(xyz_c, cfg_hi)
     q0 = np.stack([E.initial_condition(k, cfg_f, xyz_all[perm_f], seed=seed) for k in TRACERS], axis=1)
     fine.set_state(q0)
     log(f"closure data: fine {Nf}x{Nv} dt {dt_f:.1f}s, coarse {Nc}x{Nv} dt {dt_c:.1f}s, setup {time.time() - t0:.1f}s")
-    last = int(span * nsteps_c)
+    last = int(span * nsteps_c) - window
     ks = sorted({int(round(j * last / snapshots)) for j in range(snapshots)})
     feats, As, Rs = [], [], []
     done_c = 0
     for k in ks:
+        assert k >= done_c, "snapshots closer than the window"
         if k > done_c:
             fine.run(F64, (k - done_c) * ratio, dt_f, done_c * dt_c)
             done_c = k
         tk = k * dt_c
         qa = fine.state(F64)[restrict]
-        fine.run(F64, ratio, dt_f, tk)
-        done_c = k + 1
+        fine.run(F64, ratio * window, dt_f, tk)
+        done_c = k + window
         qb = fine.state(F64)[restrict]
         coarse.set_state(qa)
-        feat, lap = coarse.closure_features(tk, dt_c)
-        coarse.run(F64, 1, dt_c, tk)
-        qc = coarse.state(F64)
-        R = (qb - qc).reshape(-1)
-        A = dt_c * cfg_c.hv_coef * lap
+        feat, _ = coarse.closure_features(tk, dt_c)
+        coarse.run(F64, window, dt_c, tk)
+        q1 = coarse.state(F64)
+        coarse_hi.set_state(qa)
+        coarse_hi.run(F64, window, dt_c, tk)
+        qhi = coarse_hi.state(F64)
+        R = (qb - q1).reshape(-1)
+        A = ((qhi - q1) / (s_hi - 1.0)).reshape(-1)
         keep = np.zeros((Nc, 4, Nv), bool)
         keep[:, :train_tracers, :] = True
         keep = keep.reshape(-1)
@@ -144,9 +152,11 @@ def make_dataset(fine_level=8, coarse_level=6, Nv=30, n=55, deg=4, nv=7, snapsho
     R = np.concatenate(Rs)
     sigma = float(np.sqrt(np.mean(R ** 2)))
     meta = dict(fine_nodes=Nf, coarse_nodes=Nc, levels=Nv, dt_coarse=dt_c, dt_fine=dt_f, snapshots=ks,
-                sigma=sigma, seed=seed, hv_coef_coarse=cfg_c.hv_coef, seconds=time.time() - t0)
+                window_steps=window, s_hi=s_hi, sigma=sigma, seed=seed, hv_coef_coarse=cfg_c.hv_coef,
+                seconds=time.time() - t0)
     fine.close()
     coarse.close()
+    coarse_hi.close()
     return feat, (A / sigma).astype(np.float32), (R / sigma).astype(np.float32), meta
 
 
