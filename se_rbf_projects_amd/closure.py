@@ -198,12 +198,15 @@ def train(feat, A, R, steps=4000, batch=8192, lr=2e-3, holdout=0.1, seed=0, log_
     _, loss_tr = cl.grad(0, ntr)
     _, loss_ho = cl.grad(ntr, nho)
     s_opt, loss_opt = optimal_constant(A[ntr:], R[ntr:])
+    s_adm = min(max(s_opt, 0.0), s_max)   # the best constant the stability clamp admits
+    loss_adm = float(np.mean(((s_adm - 1.0) * A[ntr:].astype(np.float64) - R[ntr:]) ** 2))
     o = cl.eval(ntr, nho)
     sv = np.minimum(np.log1p(np.exp(np.minimum(o, 30.0))), s_max)
     report = dict(samples=int(m), train=int(ntr), holdout=int(nho), steps=steps, batch=batch, lr=lr, s_max=s_max,
                   train_seconds=secs, loss_const_train=loss0_tr, loss_const_holdout=loss0_ho,
                   loss_trained_train=loss_tr, loss_trained_holdout=loss_ho,
                   best_constant_multiplier=s_opt, loss_best_constant_holdout=loss_opt,
+                  best_admissible_constant=s_adm, loss_best_admissible_constant_holdout=loss_adm,
                   s_holdout_percentiles={q: float(np.percentile(sv, q)) for q in (1, 10, 50, 90, 99)})
     params = cl.params()
     cl.close()
