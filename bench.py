@@ -12,7 +12,7 @@ One point-update = one (node, level) advanced one RK4 step, all four tracers.
 Timed legs on the same inputs:
   f64          the plain method in FP64 (hyperviscosity in every stage)
   f32+closure  FP32 state and SpMV, hyperviscosity in every stage with the learned
-               closure's per-point coefficients (re-evaluated every 8 steps)  -> value
+               closure's per-point coefficients (re-evaluated every 32 steps)  -> value
   f32          the same with the constant coefficient (its stage kernel gives the roofline)
 --period runs both the f64 and the f32+closure legs over the full 12-day period of the
 reversing flow and states their error against the exact solution q(T) = q(0).
@@ -62,9 +62,11 @@ def load_traffic(kernel_key):
 
 
 def error_vs(q, qref):
+    """Relative l2 / linf over the three transported tracers, and l2 per tracer."""
     d = q[:, :3] - qref[:, :3]
     return {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(qref[:, :3])),
-            "rel_linf": float(np.max(np.abs(d)) / np.max(np.abs(qref[:, :3])))}
+            "rel_linf": float(np.max(np.abs(d)) / np.max(np.abs(qref[:, :3]))),
+            "rel_l2_per_tracer": [float(np.linalg.norm(d[:, i]) / np.linalg.norm(qref[:, i])) for i in range(3)]}
 
 
 def main():
@@ -79,6 +81,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=1024, help="size of the stream-copy bandwidth reference")
     ap.add_argument("--period", action="store_true", help="also run the full period and state the errors")
+    ap.add_argument("--closure-every", type=int, default=32, help="steps between closure re-evaluations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not part of "
@@ -129,7 +132,7 @@ def main():
     res, final = {}, {}
     for name, prec, split, use_cl in legs:
         eng.set_hv_split(split)
-        eng.attach_closure(closure if use_cl else None, every=8, s_max=s_max)
+        eng.attach_closure(closure if use_cl else None, every=args.closure_every, s_max=s_max)
         eng.set_state(q0)
         eng.run(prec, args.warmup, dt)
         eng.set_state(q0)
@@ -153,7 +156,7 @@ def main():
         period = {"steps": nsteps_T, "dt_s": dt, "reference": "exact solution q(T) = q(0) of the reversing flow"}
         for name, prec, split, use_cl in legs[:2]:
             eng.set_hv_split(split)
-            eng.attach_closure(closure if use_cl else None, every=8, s_max=s_max)
+            eng.attach_closure(closure if use_cl else None, every=args.closure_every, s_max=s_max)
             eng.set_state(q0)
             t = time.time()
             eng.run(prec, nsteps_T, dt)
@@ -199,7 +202,7 @@ def main():
                                "hyperviscosity",
                    "nodes": N, "levels": cfg.Nv, "tracers": ntr, "stencil": cfg.n, "dt_s": dt,
                    "point_update": "one (node, level) advanced one RK4 step (all 4 tracers)",
-                   "closure": closure_kind, "closure_every_steps": 8, "closure_s_max": s_max},
+                   "closure": closure_kind, "closure_every_steps": args.closure_every, "closure_s_max": s_max},
         "fp64_plain": {"value": value["f64"], "ms_per_step": res["f64"][0] / args.steps,
                        "stage_us": res["f64"][1] * 1e3, "roofline_achieved_GBs": achieved64,
                        "roofline_frac": achieved64 / HBM_PEAK_GBS},

@@ -220,7 +220,8 @@ int serbf_closure_eval(serbf_closure *c, long first, long count, float *o);
  * lap[...] = Delta^k q (unit sphere).  Needs config-mode weights with hv_order >= 3. */
 int serbf_closure_features(serbf_engine *e, double t, double dt, float *feat, double *lap);
 /* Use the closure in the engine's time loop: the coefficients are re-evaluated from the
- * step-start state every `every` steps (1..8), with the multiplier clamped to s_max.
+ * step-start state on the steps of each serbf_run call that are multiples of `every`
+ * (1..4096; step 0 always), with the multiplier clamped to s_max.
  * c = NULL returns to the constant coefficient.  The engine keeps a pointer to c: detach
  * before destroying it. */
 int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_max);

@@ -71,6 +71,13 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
 }
 __device__ __forceinline__ float softplus_f(float o) { return o > 20.f ? o : log1pf(expf(o)); }
 __device__ __forceinline__ float sigmoid_f(float o) { return 1.f / (1.f + expf(-o)); }
+// tanh(x) = sign(x) (1 - 2 / (exp(2|x|) + 1)) with the hardware exp and reciprocal: absolute
+// error ~1e-7, far below the BF16 rounding of the next layer's operand
+__device__ __forceinline__ float tanh_fast(float x)
+{
+    const float e = __expf(2.f * fabsf(x));
+    return copysignf(1.f - 2.f * __frcp_rn(e + 1.f), x);
+}
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -126,7 +133,7 @@ __device__ __forceinline__ void fwd_tile(const FwdRegs &r, v4s xb, float h1[4][4
     for (int t = 0; t < 4; t++) {
         const v4f z = mfma(r.w1[t], xb, zero);
 #pragma unroll
-        for (int i = 0; i < 4; i++) h1[t][i] = tanhf(z[i] + r.b1[t][i]);
+        for (int i = 0; i < 4; i++) h1[t][i] = tanh_fast(z[i] + r.b1[t][i]);
     }
     v4f z2[4] = {zero, zero, zero, zero};
 #pragma unroll
@@ -138,7 +145,7 @@ __device__ __forceinline__ void fwd_tile(const FwdRegs &r, v4s xb, float h1[4][4
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) h2[t][i] = tanhf(z2[t][i] + r.b2[t]);
+        for (int i = 0; i < 4; i++) h2[t][i] = tanh_fast(z2[t][i] + r.b2[t]);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         float s = 0.f;
@@ -235,45 +242,52 @@ __global__ void __launch_bounds__(256) features_kernel(FeatArgs<T> a, float *fea
 // closure coefficients: gamma[(p*ntr + tr)*S + v] = hv_coef * s, s = softplus(MLP(x)), with
 // s clamped to s_max, s = 1 where it is not finite; levels >= Nv get 0
 template <typename T, int S>
-__global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, T hv_coef, float s_max)
+__global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, T hv_coef, float s_max,
+                                                    int every)
 {
     constexpr int CPW = 64 / S;
     __shared__ alignas(16) unsigned short xs[4][64 * 16];
     __shared__ float os[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
-    const long wave = (long)blockIdx.x * 4 + w;
+    // re-evaluate only on steps that are multiples of `every` (the step counter lives on the
+    // device, so one captured graph serves every cadence); other steps exit at once
+    if ((*a.s.base + a.s.step_off) % every != 0) return;
     const long ncol = (long)a.s.N * a.ntr;
-    if (wave * CPW >= ncol) return;   // whole wavefronts only
-    long col = wave * CPW + lane / S;
-    const bool ok = col < ncol;
-    if (!ok) col = ncol - 1;
-    const int v = lane % S;
-    const int p = (int)(col / a.ntr), tr = (int)(col % a.ntr);
-    float f[kF];
-    T l4;
-    point_features<T, S>(a, p, tr, v, f, l4);
-    unsigned short *xrow = &xs[w][lane * 16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const float x = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
-        xrow[k] = f2bf(x);
-    }
-    wave_sync();
+    const long ngroups = (ncol + CPW - 1) / CPW;   // 64-point groups, one per wavefront pass
     FwdRegs r;
     load_fwd(nv, lane, r);
+    // grid-stride over the groups; the loop bound is wavefront-uniform
+    for (long grp = (long)blockIdx.x * 4 + w; grp < ngroups; grp += (long)gridDim.x * 4) {
+        long col = grp * CPW + lane / S;
+        const bool ok = col < ncol;
+        if (!ok) col = ncol - 1;
+        const int v = lane % S;
+        const int p = (int)(col / a.ntr), tr = (int)(col % a.ntr);
+        float f[kF];
+        T l4;
+        point_features<T, S>(a, p, tr, v, f, l4);
+        unsigned short *xrow = &xs[w][lane * 16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const float x = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
+            xrow[k] = f2bf(x);
+        }
+        wave_sync();
 #pragma unroll 1
-    for (int st = 0; st < 4; st++) {
-        const v4s xb = ld4(&xs[w][(16 * st + c) * 16 + 4 * g]);
-        float h1[4][4], h2[4][4], o[4];
-        fwd_tile(r, xb, h1, h2, o);
-        if (c < 4) os[w][16 * st + 4 * g + c] = pick4(o, c);
+        for (int st = 0; st < 4; st++) {
+            const v4s xb = ld4(&xs[w][(16 * st + c) * 16 + 4 * g]);
+            float h1[4][4], h2[4][4], o[4];
+            fwd_tile(r, xb, h1, h2, o);
+            if (c < 4) os[w][16 * st + 4 * g + c] = pick4(o, c);
+        }
+        wave_sync();
+        float sv = softplus_f(os[w][lane]);
+        if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
+        if (sv > s_max) sv = s_max;
+        if (ok) gamma[((size_t)p * a.ntr + tr) * S + v] = v < a.s.Nv ? hv_coef * (T)sv : T(0);
+        wave_sync();   // xs / os are rewritten by the next pass
     }
-    wave_sync();
-    float sv = softplus_f(os[w][lane]);
-    if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
-    if (sv > s_max) sv = s_max;
-    if (ok) gamma[((size_t)p * a.ntr + tr) * S + v] = v < a.s.Nv ? hv_coef * (T)sv : T(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -579,16 +593,24 @@ int ensure_hloc(serbf_engine *e)
 }
 
 template <typename T>
-int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma)
+int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma, int every)
 {
     const long ncol = (long)e->N * e->ntr;
-    const long waves = e->S == 16 ? (ncol + 3) / 4 : ncol;
-    const dim3 grid((unsigned)((waves + 3) / 4));
+    const long groups = e->S == 16 ? (ncol + 3) / 4 : ncol;
+    // a fixed grid of 8 blocks (32 wavefronts) per CU striding over the 64-point groups
+    if (e->cus <= 0) {
+        hipDeviceProp_t prop;
+        e->cus = hipGetDeviceProperties(&prop, e->device) == hipSuccess ? prop.multiProcessorCount : 256;
+    }
+    long blocks = (long)e->cus * 8;
+    if (blocks > (groups + 3) / 4) blocks = (groups + 3) / 4;
+    const dim3 grid((unsigned)blocks);
     const NetView nv = view(e->closure);
     const T hv = (T)e->cfg.hv_coef;
     const float smax = (float)e->closure_smax;
-    if (e->S == 16) hipLaunchKernelGGL((apply_kernel<T, 16>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax);
-    else hipLaunchKernelGGL((apply_kernel<T, 64>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax);
+    if (e->S == 16)
+        hipLaunchKernelGGL((apply_kernel<T, 16>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax, every);
+    else hipLaunchKernelGGL((apply_kernel<T, 64>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax, every);
     SERBF_HIP(hipGetLastError());
     return SERBF_OK;
 }
@@ -613,7 +635,7 @@ template <typename T>
 int closure_enqueue(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off)
 {
     const FeatArgs<T> a = feat_args<T>(e, sb, sb.qT, dt, sb.tscal, e->d_base, step_off);
-    return launch_apply<T>(e, a, sb.gamma);
+    return launch_apply<T>(e, a, sb.gamma, e->closure_every);
 }
 template int closure_enqueue<float>(serbf_engine *, StateBuffers<float> &, double, int);
 template int closure_enqueue<double>(serbf_engine *, StateBuffers<double> &, double, int);
@@ -811,7 +833,8 @@ int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_
         e->hv_mode = 0;
         return SERBF_OK;
     }
-    SERBF_CHECK(every >= 1 && every <= 8 && s_max > 0.0, SERBF_ERR_ARG, "closure_attach: every 1..8, s_max > 0");
+    SERBF_CHECK(every >= 1 && every <= 4096 && s_max > 0.0, SERBF_ERR_ARG,
+                "closure_attach: every 1..4096, s_max > 0");
     SERBF_CHECK(c->have_params, SERBF_ERR_STATE, "closure_attach: closure has no parameters");
     SERBF_CHECK(c->device == e->device, SERBF_ERR_ARG, "closure_attach: closure and engine on different devices");
     SERBF_CHECK(e->have_state && e->have_lap && e->s64.WL1 && e->s32.WL1, SERBF_ERR_STATE,
@@ -844,7 +867,7 @@ int serbf_closure_gamma(serbf_engine *e, int precision, double t, double dt, dou
         SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(tsf)));
         SERBF_HIP(hipMemcpyAsync(d_ts, tsf, sizeof(tsf), hipMemcpyHostToDevice, e->stream));
         const FeatArgs<float> a = feat_args<float>(e, e->s32, e->s32.qT, dt, d_ts, e->d_base + 1, 0);
-        rc = launch_apply<float>(e, a, e->s32.gamma);
+        rc = launch_apply<float>(e, a, e->s32.gamma, 1);
         if (!rc) {
             std::vector<float> g(P);
             SERBF_HIP(hipMemcpyAsync(g.data(), e->s32.gamma, P * sizeof(float), hipMemcpyDeviceToHost, e->stream));
@@ -858,7 +881,7 @@ int serbf_closure_gamma(serbf_engine *e, int precision, double t, double dt, dou
         SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
         SERBF_HIP(hipMemcpyAsync(d_ts, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
         const FeatArgs<double> a = feat_args<double>(e, e->s64, e->s64.qT, dt, d_ts, e->d_base + 1, 0);
-        rc = launch_apply<double>(e, a, e->s64.gamma);
+        rc = launch_apply<double>(e, a, e->s64.gamma, 1);
         if (!rc) {
             SERBF_HIP(hipMemcpy2DAsync(gamma, Nv * sizeof(double), e->s64.gamma, S * sizeof(double),
                                        Nv * sizeof(double), cols, hipMemcpyDeviceToHost, e->stream));

@@ -130,6 +130,7 @@ struct serbf_engine {
     int closure_every = 8;
     double closure_smax = 8.0;
     float *d_hloc = nullptr;   // nearest-neighbour chord per node (closure features)
+    int cus = 0;               // compute units of the device (0 until queried)
 };
 
 namespace serbf {

@@ -762,7 +762,8 @@ int enqueue_step(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off)
     a.qT = sb.qT;
     a.F = sb.F;
     int rc;
-    if (e->closure && step_off % e->closure_every == 0 && (rc = closure_enqueue<T>(e, sb, dt, step_off))) return rc;
+    // the closure kernel runs at every step and re-evaluates on multiples of closure_every
+    if (e->closure && (rc = closure_enqueue<T>(e, sb, dt, step_off))) return rc;
     a.qin = sb.qT; a.qout = sb.qB;
     if ((rc = launch_stage<T>(e, 1, hv1, a))) return rc;
     a.qin = sb.qB; a.qout = sb.qA;
