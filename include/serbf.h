@@ -132,6 +132,21 @@ int serbf_set_stencils(serbf_engine *e, int n, const int *idx);
 int serbf_configure(serbf_engine *e, const serbf_config *cfg);
 /* Batched FP64 weight solves for every node (one persistent launch). */
 int serbf_weights(serbf_engine *e);
+/* Per-node stencil sizes and polynomial degrees (config mode).  The stencil rows keep their
+ * stride n (the largest size, from serbf_knn / serbf_set_stencils); node i uses its first
+ * nn[i] neighbours (2 <= nn[i] <= n) and tangent-plane monomials of degree deg[i] (0..6, below
+ * 2 hv_order), with nn[i] + (deg+1)(deg+2)/2 <= 112; its other row entries get zero weights, so
+ * the time loop runs unchanged on the padded rows.  All the uneven systems are solved in one
+ * persistent launch by serbf_weights.  nn = deg = NULL returns to uniform stencils.  Call after
+ * serbf_configure. */
+int serbf_set_stencil_sizes(serbf_engine *e, const int *nn, const int *deg);
+/* Autotuner of the batched weight solves: times each schedule variant that fits (refinement
+ * residual re-evaluated or from an LDS copy of the matrix; node order or largest systems
+ * first) on `sample` evenly spaced nodes and keeps the fastest for serbf_weights.
+ * report[3*v + {0,1,2}] = variant, ms (-1: does not fit), systems per second; *nvariants = 4. */
+int serbf_weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants);
+/* The last serbf_weights launch: kernel time (ms), systems solved, variant used. */
+int serbf_weights_stats(serbf_engine *e, double *ms, double *systems, int *variant);
 /* status[i] = 0 ok, 1 singular gradient system, 2 singular Delta^k system */
 int serbf_get_weights(serbf_engine *e, double *Dx, double *Dy, double *L, int *status);
 int serbf_set_weights(serbf_engine *e, const double *Dx, const double *Dy, const double *L);

@@ -56,6 +56,11 @@ int orc_weights(const serbf_config *cfg, int N, const double *x, const double *y
 int orc_weights_lap(const serbf_config *cfg, int N, const double *x, const double *y,
                     const double *z, const int *idx, double *Dx, double *Dy, double *L,
                     double *L1, double *L2, int *status);
+/* Config mode with per-node stencil size nn[i] <= n and polynomial degree deg[i]; rows keep
+ * the stride n, entries past nn[i] are zero. */
+int orc_weights_var(const serbf_config *cfg, int N, const double *x, const double *y,
+                    const double *z, const int *idx, const int *nn, const int *deg, double *Dx,
+                    double *Dy, double *L, int *status);
 /* Vertical operator: start[Nv], W[Nv*nv]; returns nv. */
 int orc_vertical(const serbf_config *cfg, int *start, double *W);
 

@@ -376,6 +376,40 @@ int orc_weights_lap(const serbf_config *cfg, int N, const double *x, const doubl
     return bad;
 }
 
+/* Config mode with a stencil size nn[i] <= cfg->n and polynomial degree deg[i] per node; the
+ * stencil and weight rows keep the stride cfg->n, entries past nn[i] get zero weights. */
+int orc_weights_var(const serbf_config *cfg, int N, const double *x, const double *y,
+                    const double *z, const int *idx, const int *nn, const int *deg, double *Dx,
+                    double *Dy, double *L, int *status)
+{
+    const int stride = cfg->n;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 32) reduction(+ : bad)
+    for (int i = 0; i < N; i++) {
+        serbf_config c = *cfg;
+        const int n = nn[i];
+        c.n = n;
+        c.poly_deg = deg[i];
+        double X[3 * 512];
+        double H[9];
+        const int *st = idx + (size_t)i * stride;
+        for (int s = 0; s < n; s++) {
+            X[s] = x[st[s]];
+            X[n + s] = y[st[s]];
+            X[2 * n + s] = z[st[s]];
+        }
+        orc_householder(x[i], y[i], z[i], 1, H);
+        double *dx = Dx + (size_t)i * stride, *dy = Dy + (size_t)i * stride, *dl = L + (size_t)i * stride;
+        int info2 = 0;
+        const int info1 = cfg_node_weights(&c, n, X, H, dx, dy, dl, NULL, NULL, &info2);
+        for (int s = n; s < stride; s++)
+            dx[s] = dy[s] = dl[s] = 0.0;
+        status[i] = info1 ? 1 : (info2 ? 2 : 0);
+        bad += status[i] != 0;
+    }
+    return bad;
+}
+
 /* Vertical operator.  Reference mode: 7-point 6th-order FD1 / dh on levels v-3..v+3
  * with zero values outside the column (config.h:21-22, rbffd_DMs.c:146-154,412-427).
  * Config mode: 1D PHS |z|^kv + degree-dv polynomials on the nv levels nearest to v

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -94,6 +96,12 @@ struct serbf_engine {
     // config mode: Delta and Delta^2 weights (closure features), filled by the weight solves
     double *d_WL1 = nullptr, *d_WL2 = nullptr;
     bool have_lap = false;
+    // per-node stencil sizes / polynomial degrees (empty: uniform), the weight-solve schedule
+    std::vector<int> h_nn, h_deg;
+    int *d_nn = nullptr, *d_deg = nullptr, *d_worder = nullptr;
+    int wvariant = 0;            // schedule variant of the weight solves (autotuner)
+    double wstats_ms = 0.0, wstats_systems = 0.0;
+    int wstats_variant = -1;
     int *d_status = nullptr;
     bool have_weights = false;
     // time loop buffers
@@ -142,6 +150,7 @@ int convert_to_f32(serbf_engine *e, size_t n, const double *src, float *dst);
 int knn_run(serbf_engine *e, int n);
 // weights (weights.hip)
 int weights_run(serbf_engine *e);
+int weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants);
 int vertical_table(const serbf_config &cfg, int *start, double *W);
 // time loop (rk4.hip)
 int rk4_prepare(serbf_engine *e);

@@ -89,6 +89,8 @@ int serbf_create(serbf_engine **out, int device)
     return SERBF_OK;
 }
 
+static void clear_sizes(serbf_engine *e);
+
 static void free_nodes(serbf_engine *e)
 {
     for (double *p : {e->d_x, e->d_y, e->d_z, e->d_lam, e->d_phi, e->d_D, e->d_Wx, e->d_Wy, e->d_WL, e->d_WL1,
@@ -97,6 +99,7 @@ static void free_nodes(serbf_engine *e)
     for (int *p : {e->d_idx, e->d_status, e->d_vstart, e->d_tuni, e->d_tcount})
         if (p) (void)hipFree(p);
     if (e->d_lidx) (void)hipFree(e->d_lidx);
+    clear_sizes(e);
     if (e->d_hloc) (void)hipFree(e->d_hloc);
     e->d_hloc = nullptr;
     e->closure = nullptr;
@@ -162,9 +165,19 @@ int serbf_knn(serbf_engine *e, int n)
     return knn_run(e, n);
 }
 
+static void clear_sizes(serbf_engine *e)
+{
+    for (int *p : {e->d_nn, e->d_deg, e->d_worder})
+        if (p) (void)hipFree(p);
+    e->d_nn = e->d_deg = e->d_worder = nullptr;
+    e->h_nn.clear();
+    e->h_deg.clear();
+}
+
 static int alloc_stencils_impl(serbf_engine *e, int n)
 {
     e->have_plan = false;
+    clear_sizes(e);
     if (e->d_idx && e->n != n) {
         (void)hipFree(e->d_idx);
         (void)hipFree(e->d_D);
@@ -313,6 +326,55 @@ int serbf_set_weights(serbf_engine *e, const double *Dx, const double *Dy, const
     SERBF_HIP(hipStreamSynchronize(e->stream));
     e->have_weights = true;
     e->have_lap = false;
+    return SERBF_OK;
+}
+
+int serbf_set_stencil_sizes(serbf_engine *e, const int *nn, const int *deg)
+{
+    SERBF_CHECK(e && e->have_stencils && e->configured, SERBF_ERR_STATE,
+                "set_stencil_sizes: stencils and configuration first");
+    SERBF_HIP(hipSetDevice(e->device));
+    clear_sizes(e);
+    e->have_weights = false;
+    rk4_free(e);
+    e->have_state = false;
+    if (!nn && !deg) return SERBF_OK;
+    SERBF_CHECK(nn && deg, SERBF_ERR_ARG, "set_stencil_sizes: both arrays or neither");
+    SERBF_CHECK(e->cfg.mode == SERBF_MODE_CONFIG, SERBF_ERR_UNSUPPORTED, "set_stencil_sizes: config mode only");
+    for (int i = 0; i < e->N; i++) {
+        SERBF_CHECK(nn[i] >= 2 && nn[i] <= e->n, SERBF_ERR_ARG, "set_stencil_sizes: nn[i] must be in [2, n]");
+        SERBF_CHECK(deg[i] >= 0 && deg[i] <= 6 && (e->cfg.hv_order == 0 || deg[i] < 2 * e->cfg.hv_order),
+                    SERBF_ERR_ARG, "set_stencil_sizes: deg[i] must be in [0, 6] and below 2*hv_order");
+        SERBF_CHECK(nn[i] + (deg[i] + 1) * (deg[i] + 2) / 2 <= 112, SERBF_ERR_UNSUPPORTED,
+                    "set_stencil_sizes: nn + polynomial terms must be <= 112");
+    }
+    e->h_nn.assign(nn, nn + e->N);
+    e->h_deg.assign(deg, deg + e->N);
+    SERBF_HIP(hipMalloc((void **)&e->d_nn, e->N * sizeof(int)));
+    SERBF_HIP(hipMalloc((void **)&e->d_deg, e->N * sizeof(int)));
+    SERBF_HIP(hipMemcpyAsync(e->d_nn, nn, e->N * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    SERBF_HIP(hipMemcpyAsync(e->d_deg, deg, e->N * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    return SERBF_OK;
+}
+
+int serbf_weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants)
+{
+    SERBF_CHECK(e && e->configured && e->have_stencils, SERBF_ERR_STATE, "weights_autotune: configure and stencils first");
+    SERBF_CHECK(e->cfg.n == e->n, SERBF_ERR_ARG, "weights_autotune: config n differs from the stencil size");
+    SERBF_HIP(hipSetDevice(e->device));
+    e->have_weights = false;
+    rk4_free(e);
+    e->have_state = false;
+    return weights_autotune(e, sample, report, nvariants);
+}
+
+int serbf_weights_stats(serbf_engine *e, double *ms, double *systems, int *variant)
+{
+    SERBF_CHECK(e && e->wstats_variant >= 0, SERBF_ERR_STATE, "weights_stats: no weight solve yet");
+    if (ms) *ms = e->wstats_ms;
+    if (systems) *systems = e->wstats_systems;
+    if (variant) *variant = e->wstats_variant;
     return SERBF_OK;
 }
 

@@ -21,6 +21,7 @@
 //     hyperviscosity operator (weights / s^2k).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -40,6 +41,9 @@ struct WArgs {
     double *WL1, *WL2;   // config mode: Delta and Delta^2 weights from the gradient system (may be null)
     int *status;
     int *counter;
+    const int *nn, *dg;  // per-node stencil size and polynomial degree (null: n, deg for every node)
+    const int *order;    // work item -> node (null: identity)
+    int count;           // work items
 };
 
 // x^k for a small integer k >= 0 by repeated squaring (the PHS orders and monomial degrees)
@@ -85,11 +89,12 @@ __device__ __forceinline__ void wave_sync()
 // right-hand-side columns per system (gradient system: d/dx', d/dy', Delta, Delta^2)
 constexpr int kRhs = 4;
 
-template <int MAXM>
+//   A0  [MAXM][MAXM] (COPY variant) the assembled matrix, for the refinement residual
+template <int MAXM, bool COPY>
 struct Lds {
     static constexpr int LD = ((MAXM + 4) | 1);
     static constexpr int MAXN = MAXM;
-    static constexpr int doubles = MAXM * LD + 3 * MAXN + 2 * MAXM * kRhs;
+    static constexpr int doubles = MAXM * LD + 3 * MAXN + 2 * MAXM * kRhs + (COPY ? MAXM * MAXM : 0);
     static constexpr int bytes = doubles * 8 + MAXM * 4;
 };
 
@@ -117,11 +122,11 @@ __device__ __forceinline__ double entry(int i, int j, int n, int m, int kind, co
     return ipow(X[node], p - q) * ipow(X[MAXN + node], q);
 }
 
-template <int MAXM>
-__device__ int factor_solve(double *A, double *X, double *B0, double *XS, int *piv, int n, int m, int nrhs,
-                            int kind, const WArgs &a, int lane)
+template <int MAXM, bool COPY>
+__device__ int factor_solve(double *A, double *X, double *B0, double *XS, const double *A0, int *piv, int n, int m,
+                            int nrhs, int kind, const WArgs &a, int lane)
 {
-    constexpr int LD = Lds<MAXM>::LD, MAXN = Lds<MAXM>::MAXN, R = (MAXM + 63) / 64;
+    constexpr int LD = Lds<MAXM, COPY>::LD, MAXN = Lds<MAXM, COPY>::MAXN, R = (MAXM + 63) / 64;
     int info = 0;
     // ---- LU with partial pivoting on [A | B] ----
     for (int k = 0; k < m; k++) {
@@ -211,7 +216,7 @@ __device__ int factor_solve(double *A, double *X, double *B0, double *XS, int *p
             double acc[kRhs];
             for (int c = 0; c < kRhs; c++) acc[c] = B0[i * kRhs + c];
             for (int j = 0; j < m; j++) {
-                const double aij = entry(i, j, n, m, kind, a, X, MAXN);
+                const double aij = COPY ? A0[i * MAXM + j] : entry(i, j, n, m, kind, a, X, MAXN);
                 for (int c = 0; c < nrhs; c++) acc[c] -= aij * XS[j * kRhs + c];
             }
             for (int c = 0; c < nrhs; c++) res[r][c] = acc[c];
@@ -265,13 +270,17 @@ __device__ int factor_solve(double *A, double *X, double *B0, double *XS, int *p
     return 0;
 }
 
-template <int MAXM>
-__device__ void assemble(double *A, double *X, double *B0, int n, int m, int nrhs, int kind, const WArgs &a,
-                         int lane)
+template <int MAXM, bool COPY>
+__device__ void assemble(double *A, double *X, double *B0, double *A0, int n, int m, int nrhs, int kind,
+                         const WArgs &a, int lane)
 {
-    constexpr int LD = Lds<MAXM>::LD, MAXN = Lds<MAXM>::MAXN;
+    constexpr int LD = Lds<MAXM, COPY>::LD, MAXN = Lds<MAXM, COPY>::MAXN;
     for (int i = lane; i < m; i += 64) {
-        for (int j = 0; j < m; j++) A[i * LD + j] = entry(i, j, n, m, kind, a, X, MAXN);
+        for (int j = 0; j < m; j++) {
+            const double v = entry(i, j, n, m, kind, a, X, MAXN);
+            A[i * LD + j] = v;
+            if (COPY) A0[i * MAXM + j] = v;
+        }
         double b[kRhs] = {0.0, 0.0, 0.0, 0.0};
         if (i < n) {
             // derivative at the centre (stencil node 0) of the basis function centred at node i
@@ -316,31 +325,35 @@ __device__ void assemble(double *A, double *X, double *B0, int n, int m, int nrh
     wave_sync();
 }
 
-template <int MAXM>
+template <int MAXM, bool COPY>
 __global__ void __launch_bounds__(64) weights_kernel(WArgs a)
 {
     extern __shared__ double smem[];
-    constexpr int LD = Lds<MAXM>::LD, MAXN = Lds<MAXM>::MAXN;
+    constexpr int LD = Lds<MAXM, COPY>::LD, MAXN = Lds<MAXM, COPY>::MAXN;
     double *A = smem;
     double *X = A + MAXM * LD;
     double *B0 = X + 3 * MAXN;
     double *XS = B0 + MAXM * kRhs;
-    int *piv = (int *)(XS + MAXM * kRhs);
+    double *A0 = XS + MAXM * kRhs;   // used by the COPY variant only
+    int *piv = (int *)(A0 + (COPY ? MAXM * MAXM : 0));
     const int lane = threadIdx.x;
-    const int n = a.n;
-    const int M = a.ref_mode ? 21 : (a.deg + 1) * (a.deg + 2) / 2;
-    const int m1 = n + M;
+    const int stride = a.n;   // row stride of the stencil and weight arrays
     const bool gauss_hv = a.ref_mode || a.hvphs == 0;
-    const int m2 = gauss_hv ? n + 1 : n + M;
     for (;;) {
-        int node = 0;
-        if (lane == 0) node = atomicAdd(a.counter, 1);
-        node = __builtin_amdgcn_readfirstlane(__shfl(node, 0, 64));
-        if (node >= a.N) break;
+        int item = 0;
+        if (lane == 0) item = atomicAdd(a.counter, 1);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= a.count) break;
+        const int node = a.order ? a.order[item] : item;
+        const int n = a.nn ? a.nn[node] : stride;
+        const int deg = a.dg ? a.dg[node] : a.deg;
+        const int M = a.ref_mode ? 21 : (deg + 1) * (deg + 2) / 2;
+        const int m1 = n + M;
+        const int m2 = gauss_hv ? n + 1 : n + M;
         double H[9];
         householder_dev(a.x[node], a.y[node], a.z[node], !a.ref_mode, H);
         // gather and rotate the stencil (rbffd_DMs.c:159-168)
-        const int *ip = a.idx + (size_t)node * n;
+        const int *ip = a.idx + (size_t)node * stride;
         for (int s = lane; s < n; s += 64) {
             const int q = ip[s];
             const double px = a.x[q], py = a.y[q], pz = a.z[q];
@@ -368,19 +381,22 @@ __global__ void __launch_bounds__(64) weights_kernel(WArgs a)
             wave_sync();
         }
         int st = 0;
+        const size_t row = (size_t)node * stride;
         // gradient system (config mode with closure weights: also Delta and Delta^2)
         const int ng = a.WL1 ? 4 : 2;
-        assemble<MAXM>(A, X, B0, n, m1, ng, 0, a, lane);
-        if (factor_solve<MAXM>(A, X, B0, XS, piv, n, m1, ng, 0, a, lane)) st = 1;
-        for (int s = lane; s < n; s += 64) {
-            a.Wx[(size_t)node * n + s] = st ? 0.0 : XS[s * kRhs] / scale;
-            a.Wy[(size_t)node * n + s] = st ? 0.0 : XS[s * kRhs + 1] / scale;
+        assemble<MAXM, COPY>(A, X, B0, A0, n, m1, ng, 0, a, lane);
+        if (factor_solve<MAXM, COPY>(A, X, B0, XS, A0, piv, n, m1, ng, 0, a, lane)) st = 1;
+        for (int s = lane; s < stride; s += 64) {
+            const bool in = s < n && !st;
+            a.Wx[row + s] = in ? XS[s * kRhs] / scale : 0.0;
+            a.Wy[row + s] = in ? XS[s * kRhs + 1] / scale : 0.0;
         }
         if (a.WL1) {
             const double s2 = scale * scale, s4 = s2 * s2;
-            for (int s = lane; s < n; s += 64) {
-                a.WL1[(size_t)node * n + s] = st ? 0.0 : XS[s * kRhs + 2] / s2;
-                a.WL2[(size_t)node * n + s] = st ? 0.0 : XS[s * kRhs + 3] / s4;
+            for (int s = lane; s < stride; s += 64) {
+                const bool in = s < n && !st;
+                a.WL1[row + s] = in ? XS[s * kRhs + 2] / s2 : 0.0;
+                a.WL2[row + s] = in ? XS[s * kRhs + 3] / s4 : 0.0;
             }
         }
         wave_sync();
@@ -389,50 +405,92 @@ __global__ void __launch_bounds__(64) weights_kernel(WArgs a)
             // the Gaussian shape parameter of the unit sphere, in the scaled solve frame
             WArgs ag = a;
             ag.eps = a.eps * scale;
-            assemble<MAXM>(A, X, B0, n, m2, 1, 1, ag, lane);
-            int bad = factor_solve<MAXM>(A, X, B0, XS, piv, n, m2, 1, 1, ag, lane);
+            assemble<MAXM, COPY>(A, X, B0, A0, n, m2, 1, 1, ag, lane);
+            int bad = factor_solve<MAXM, COPY>(A, X, B0, XS, A0, piv, n, m2, 1, 1, ag, lane);
             if (bad && !st) st = 2;
             const double s2k = pow(scale, 2.0 * a.hv);
-            for (int s = lane; s < n; s += 64) a.WL[(size_t)node * n + s] = bad ? 0.0 : XS[s * kRhs] / s2k;
+            for (int s = lane; s < stride; s += 64) a.WL[row + s] = (s < n && !bad) ? XS[s * kRhs] / s2k : 0.0;
         } else {
-            for (int s = lane; s < n; s += 64) a.WL[(size_t)node * n + s] = 0.0;
+            for (int s = lane; s < stride; s += 64) a.WL[row + s] = 0.0;
         }
         if (lane == 0) a.status[node] = st;
         wave_sync();
     }
 }
 
-template <int MAXM>
-static int launch_weights(serbf_engine *e, const WArgs &a)
+namespace {
+
+template <int MAXM, bool COPY>
+int launch_weights(serbf_engine *e, const WArgs &a, int blocks_per_cu)
 {
-    const int bytes = Lds<MAXM>::bytes;
-    SERBF_HIP(hipFuncSetAttribute((const void *)weights_kernel<MAXM>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  bytes));
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, e->device) == hipSuccess) cus = prop.multiProcessorCount;
-    const int per_cu = (160 * 1024) / bytes > 0 ? (160 * 1024) / bytes : 1;
-    int grid = cus * per_cu;
-    if (grid > e->N) grid = e->N;
-    hipLaunchKernelGGL(weights_kernel<MAXM>, dim3(grid), dim3(64), bytes, e->stream, a);
+    const int bytes = Lds<MAXM, COPY>::bytes;
+    SERBF_HIP(hipFuncSetAttribute((const void *)weights_kernel<MAXM, COPY>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    if (e->cus <= 0) {
+        hipDeviceProp_t prop;
+        e->cus = hipGetDeviceProperties(&prop, e->device) == hipSuccess ? prop.multiProcessorCount : 256;
+    }
+    const int fit = (160 * 1024) / bytes > 0 ? (160 * 1024) / bytes : 1;
+    const int per_cu = blocks_per_cu > 0 && blocks_per_cu < fit ? blocks_per_cu : fit;
+    int grid = e->cus * per_cu;
+    if (grid > a.count) grid = a.count;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL((weights_kernel<MAXM, COPY>), dim3(grid), dim3(64), bytes, e->stream, a);
     SERBF_HIP(hipGetLastError());
     return SERBF_OK;
 }
 
-int weights_run(serbf_engine *e)
+// Schedule variants of the persistent solve: the LDS class (smallest of 48/80/112 rows that
+// holds the largest system), the residual of the refinement step (re-evaluated from the
+// coordinates, or read from a copy of the matrix kept in LDS, when it fits), and the work
+// order (node order, or largest systems first).
+struct WVariant {
+    int copy;      // 1: matrix copy in LDS
+    int sorted;    // 1: largest systems first
+};
+constexpr WVariant kWVariants[4] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+
+bool variant_fits(int maxm, int copy)
+{
+    const int bytes = maxm <= 48 ? (copy ? Lds<48, true>::bytes : Lds<48, false>::bytes)
+                      : maxm <= 80 ? (copy ? Lds<80, true>::bytes : Lds<80, false>::bytes)
+                                   : (copy ? Lds<112, true>::bytes : Lds<112, false>::bytes);
+    return bytes <= 160 * 1024;
+}
+
+int launch_class(serbf_engine *e, const WArgs &a, int maxm, int copy)
+{
+    if (maxm <= 48) return copy ? launch_weights<48, true>(e, a, 0) : launch_weights<48, false>(e, a, 0);
+    if (maxm <= 80) return copy ? launch_weights<80, true>(e, a, 0) : launch_weights<80, false>(e, a, 0);
+    return copy ? launch_weights<112, true>(e, a, 0) : launch_weights<112, false>(e, a, 0);
+}
+
+// largest system size over the nodes and the order of the nodes by decreasing size
+int system_sizes(serbf_engine *e, int &maxm, std::vector<int> &by_size)
 {
     const serbf_config &c = e->cfg;
-    const int n = e->n;
-    const int M = c.mode == SERBF_MODE_REFERENCE ? 21 : (c.poly_deg + 1) * (c.poly_deg + 2) / 2;
-    const int m = n + M;
-    SERBF_CHECK(m <= 112, SERBF_ERR_UNSUPPORTED, "weights: n + polynomial terms must be <= 112");
-    int rc = alloc_weights(e);
-    if (rc) return rc;
-    rc = ensure_scratch(e, 64);
-    if (rc) return rc;
-    SERBF_HIP(hipMemsetAsync(e->d_scratch, 0, sizeof(int), e->stream));
+    const int N = e->N;
+    std::vector<int> m(N);
+    if (e->h_nn.empty()) {
+        const int M = c.mode == SERBF_MODE_REFERENCE ? 21 : (c.poly_deg + 1) * (c.poly_deg + 2) / 2;
+        for (int i = 0; i < N; i++) m[i] = e->n + M;
+    } else {
+        for (int i = 0; i < N; i++) m[i] = e->h_nn[i] + (e->h_deg[i] + 1) * (e->h_deg[i] + 2) / 2;
+    }
+    maxm = 0;
+    for (int v : m) maxm = v > maxm ? v : maxm;
+    by_size.resize(N);
+    for (int i = 0; i < N; i++) by_size[i] = i;
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int x, int y) { return m[x] > m[y]; });
+    return SERBF_OK;
+}
+
+WArgs make_args(serbf_engine *e)
+{
+    const serbf_config &c = e->cfg;
     WArgs a;
-    a.N = e->N; a.n = n; a.ref_mode = c.mode == SERBF_MODE_REFERENCE;
+    std::memset(&a, 0, sizeof(a));
+    a.N = e->N; a.n = e->n; a.ref_mode = c.mode == SERBF_MODE_REFERENCE;
     a.phs = c.phs_order; a.deg = c.poly_deg; a.hv = c.hv_order; a.hvphs = c.hv_phs_order;
     a.eps = c.hv_eps;
     a.x = e->d_x; a.y = e->d_y; a.z = e->d_z;
@@ -443,15 +501,70 @@ int weights_run(serbf_engine *e)
     a.WL2 = lap ? e->d_WL2 : nullptr;
     a.status = e->d_status;
     a.counter = (int *)e->d_scratch;
-    if (m <= 48) rc = launch_weights<48>(e, a);
-    else if (m <= 80) rc = launch_weights<80>(e, a);
-    else rc = launch_weights<112>(e, a);
+    a.nn = e->h_nn.empty() ? nullptr : e->d_nn;
+    a.dg = e->h_nn.empty() ? nullptr : e->d_deg;
+    a.order = nullptr;
+    a.count = e->N;
+    return a;
+}
+
+// one timed launch of variant v over `count` work items of `order` (device array)
+int run_variant(serbf_engine *e, int v, int maxm, const int *d_order, int count, float *ms)
+{
+    WArgs a = make_args(e);
+    a.order = d_order;
+    a.count = count;
+    SERBF_HIP(hipMemsetAsync(e->d_scratch, 0, sizeof(int), e->stream));
+    SERBF_HIP(hipEventRecord(e->ev0, e->stream));
+    int rc = launch_class(e, a, maxm, kWVariants[v].copy);
     if (rc) return rc;
+    SERBF_HIP(hipEventRecord(e->ev1, e->stream));
+    SERBF_HIP(hipEventSynchronize(e->ev1));
+    SERBF_HIP(hipEventElapsedTime(ms, e->ev0, e->ev1));
+    return SERBF_OK;
+}
+
+int upload_order(serbf_engine *e, const std::vector<int> &order)
+{
+    if (e->d_worder) (void)hipFree(e->d_worder);
+    e->d_worder = nullptr;
+    SERBF_HIP(hipMalloc((void **)&e->d_worder, order.size() * sizeof(int)));
+    SERBF_HIP(hipMemcpyAsync(e->d_worder, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
+                             e->stream));
+    return SERBF_OK;
+}
+
+}  // namespace
+
+int weights_run(serbf_engine *e)
+{
+    int maxm = 0;
+    std::vector<int> by_size;
+    int rc = system_sizes(e, maxm, by_size);
+    if (rc) return rc;
+    SERBF_CHECK(maxm <= 112, SERBF_ERR_UNSUPPORTED, "weights: n + polynomial terms must be <= 112");
+    rc = alloc_weights(e);
+    if (rc) return rc;
+    rc = ensure_scratch(e, 64);
+    if (rc) return rc;
+    int v = e->wvariant;
+    if (v < 0 || v > 3 || !variant_fits(maxm, kWVariants[v].copy)) v = 0;
+    const int *d_order = nullptr;
+    if (kWVariants[v].sorted) {
+        if ((rc = upload_order(e, by_size))) return rc;
+        d_order = e->d_worder;
+    }
+    float ms = 0.f;
+    if ((rc = run_variant(e, v, maxm, d_order, e->N, &ms))) return rc;
+    e->wstats_ms = ms;
+    e->wstats_systems = (double)e->N * (e->cfg.hv_order > 0 ? 2.0 : 1.0);
+    e->wstats_variant = v;
     std::vector<int> st(e->N);
     SERBF_HIP(hipMemcpyAsync(st.data(), e->d_status, e->N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     SERBF_HIP(hipStreamSynchronize(e->stream));
+    const serbf_config &c = e->cfg;
     e->have_weights = true;
-    e->have_lap = lap;
+    e->have_lap = c.mode == SERBF_MODE_CONFIG && c.poly_deg >= 2;
     rk4_free(e);
     e->have_state = false;
     for (int s : st)
@@ -459,6 +572,58 @@ int weights_run(serbf_engine *e)
             set_error("weights: at least one singular system (see status)");
             return SERBF_ERR_SINGULAR;
         }
+    return SERBF_OK;
+}
+
+// Times every variant that fits on the same sample of nodes (every N/sample-th node) and keeps
+// the fastest for serbf_weights.  report[3 v] = variant, ms, systems per second (ms < 0: does
+// not fit).
+int weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants)
+{
+    int maxm = 0;
+    std::vector<int> by_size;
+    int rc = system_sizes(e, maxm, by_size);
+    if (rc) return rc;
+    SERBF_CHECK(maxm <= 112, SERBF_ERR_UNSUPPORTED, "weights: n + polynomial terms must be <= 112");
+    rc = alloc_weights(e);
+    if (rc) return rc;
+    rc = ensure_scratch(e, 64);
+    if (rc) return rc;
+    if (sample <= 0 || sample > e->N) sample = e->N;
+    // the sample: evenly spaced nodes; its sorted form keeps the same nodes, largest first
+    std::vector<int> pick(sample), rank(e->N);
+    for (int i = 0; i < sample; i++) pick[i] = (int)((long long)i * e->N / sample);
+    for (int i = 0; i < e->N; i++) rank[by_size[i]] = i;
+    std::vector<int> sorted_pick(pick);
+    std::stable_sort(sorted_pick.begin(), sorted_pick.end(), [&](int x, int y) { return rank[x] < rank[y]; });
+    int *d_pick = nullptr, *d_sorted = nullptr;
+    SERBF_HIP(hipMalloc((void **)&d_pick, sample * sizeof(int)));
+    SERBF_HIP(hipMalloc((void **)&d_sorted, sample * sizeof(int)));
+    SERBF_HIP(hipMemcpyAsync(d_pick, pick.data(), sample * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    SERBF_HIP(hipMemcpyAsync(d_sorted, sorted_pick.data(), sample * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    const double systems = (double)sample * (e->cfg.hv_order > 0 ? 2.0 : 1.0);
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < 4; v++) {
+        float ms = -1.f;
+        if (variant_fits(maxm, kWVariants[v].copy)) {
+            // one warm-up launch, then the timed one
+            rc = run_variant(e, v, maxm, kWVariants[v].sorted ? d_sorted : d_pick, sample, &ms);
+            if (!rc) rc = run_variant(e, v, maxm, kWVariants[v].sorted ? d_sorted : d_pick, sample, &ms);
+            if (rc) break;
+            if (ms < best_ms) { best_ms = ms; best = v; }
+        }
+        if (report) {
+            report[3 * v] = v;
+            report[3 * v + 1] = ms;
+            report[3 * v + 2] = ms > 0.f ? systems / (ms * 1e-3) : 0.0;
+        }
+    }
+    (void)hipFree(d_pick);
+    (void)hipFree(d_sorted);
+    if (rc) return rc;
+    if (nvariants) *nvariants = 4;
+    e->wvariant = best;
     return SERBF_OK;
 }
 

@@ -114,6 +114,22 @@ def config_deform(N, Nv=60, n=55, deg=4, nv=7, hv="gauss", gamma_c=None):
                        **_hv_fields(N, hv))
 
 
+def variable_stencil_sizes(D, n_min=31, n_max=84, k=10):
+    """Per-node stencil size from local node density and the polynomial degree that goes with
+    it.  Density is (1 / r_k)^2 with r_k the distance to the k-th neighbour (D: sorted stencil
+    distances [N, >k]); sizes spread over [n_min, n_max] by the rank of the density (the
+    densest nodes get the largest stencils).  Degree 3 below 45 nodes, 4 below 63, else 5
+    (about three nodes per polynomial term)."""
+    rk = np.asarray(D)[:, k]
+    order = np.argsort(rk, kind="stable")[::-1]   # sparsest (largest r_k) first
+    rank = np.empty(len(rk))
+    rank[order] = np.arange(len(rk))
+    frac = rank / max(len(rk) - 1, 1)
+    nn = np.rint(n_min + (n_max - n_min) * frac).astype(np.int32)
+    deg = np.where(nn < 45, 3, np.where(nn < 63, 4, 5)).astype(np.int32)
+    return nn, deg
+
+
 def courant_dt(cfg, N, courant=0.5):
     """dt = courant * h R / u_max with u_max the largest horizontal speed of the flow."""
     h = hex_spacing(N)

@@ -70,6 +70,9 @@ def lib():
             "serbf_set_tile_chunk": ([vp, i], i),
             "serbf_tile_info": ([vp, vp, vp], i),
             "serbf_get_lap_weights": ([vp, vp, vp], i),
+            "serbf_set_stencil_sizes": ([vp, vp, vp], i),
+            "serbf_weights_autotune": ([vp, i, vp, vp], i),
+            "serbf_weights_stats": ([vp, vp, vp, vp], i),
             "serbf_closure_create": ([C.POINTER(vp), i], i),
             "serbf_closure_destroy": ([vp], None),
             "serbf_closure_set_params": ([vp, vp, vp], i),
@@ -256,6 +259,29 @@ class Engine:
         if not (allow_singular and rc == ERR_SINGULAR):
             _check(rc, "weights")
         return rc
+
+    def set_stencil_sizes(self, nn, deg):
+        """Per-node stencil size and polynomial degree (None, None: uniform)."""
+        if nn is None:
+            _check(lib().serbf_set_stencil_sizes(self._e, None, None), "set_stencil_sizes")
+            return
+        self._nn = np.ascontiguousarray(nn, np.int32)
+        self._deg = np.ascontiguousarray(deg, np.int32)
+        _check(lib().serbf_set_stencil_sizes(self._e, _p(self._nn), _p(self._deg)), "set_stencil_sizes")
+
+    def weights_autotune(self, sample=16384):
+        """Times the weight-solve schedule variants on a node sample; returns
+        [(variant, ms, systems/s)] and keeps the fastest for weights()."""
+        rep = np.zeros(12)
+        nv = C.c_int()
+        _check(lib().serbf_weights_autotune(self._e, int(sample), _p(rep), C.byref(nv)), "weights_autotune")
+        return [(int(rep[3 * k]), float(rep[3 * k + 1]), float(rep[3 * k + 2])) for k in range(nv.value)]
+
+    def weights_stats(self):
+        """(kernel ms, systems solved, variant) of the last weights() call."""
+        ms, sy, v = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().serbf_weights_stats(self._e, C.byref(ms), C.byref(sy), C.byref(v)), "weights_stats")
+        return ms.value, sy.value, v.value
 
     def get_weights(self):
         Dx, Dy, L = (np.empty((self.N, self.n)) for _ in range(3))

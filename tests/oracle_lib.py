@@ -29,6 +29,7 @@ def lib():
         _lib.orc_dgesv.restype = C.c_int
         _lib.orc_weights.restype = C.c_int
         _lib.orc_weights_lap.restype = C.c_int
+        _lib.orc_weights_var.restype = C.c_int
         _lib.orc_bf16.argtypes = [C.c_float]
         _lib.orc_bf16.restype = C.c_float
         _lib.orc_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
@@ -124,6 +125,22 @@ def weights_lap(cfg, xyz, idx):
     lib().orc_weights_lap(C.byref(cfg), C.c_int(N), as_ptr(x), as_ptr(y), as_ptr(z), as_ptr(idx, C.c_int),
                           as_ptr(Dx), as_ptr(Dy), as_ptr(L), as_ptr(L1), as_ptr(L2), as_ptr(st, C.c_int))
     return Dx, Dy, L, L1, L2, st
+
+
+def weights_var(cfg, xyz, idx, nn, deg):
+    """Config-mode weights with per-node stencil size and degree (rows of stride cfg.n)."""
+    xyz = _c(xyz, np.float64)
+    idx = _c(idx, np.int32)
+    N, n = idx.shape
+    x, y, z = (np.ascontiguousarray(xyz[:, i]) for i in range(3))
+    Dx, Dy, L = (np.zeros((N, n)) for _ in range(3))
+    st = np.zeros(N, np.int32)
+    nn = _c(nn, np.int32)
+    deg = _c(deg, np.int32)
+    lib().orc_weights_var(C.byref(cfg), C.c_int(N), as_ptr(x), as_ptr(y), as_ptr(z), as_ptr(idx, C.c_int),
+                          as_ptr(nn, C.c_int), as_ptr(deg, C.c_int), as_ptr(Dx), as_ptr(Dy), as_ptr(L),
+                          as_ptr(st, C.c_int))
+    return Dx, Dy, L, st
 
 
 def vertical(cfg):

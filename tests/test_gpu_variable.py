@@ -1,0 +1,122 @@
+"""Batched uneven weight solves on the GPU: jittered nodes, stencil sizes 31..84 set by local
+density and polynomial degrees 3..5, all systems in one persistent launch with the schedule the
+autotuner picks; checked against the oracle, against LAPACK (numpy.linalg.solve) on sampled
+nodes, and through the time loop on the padded stencil rows."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from se_rbf_projects_amd import engine as E
+from se_rbf_projects_amd.ctypes_api import F32, F64, config_deform, courant_dt, variable_stencil_sizes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def var():
+    xyz = E.icos_nodes(5)
+    xyz = E.jitter_nodes(xyz, 0.3, 7)
+    xyz = xyz[E.spatial_order(xyz)]
+    N = len(xyz)
+    cfg = config_deform(N, Nv=10, n=84, deg=4, nv=5)
+    eng = E.Engine(0)
+    eng.set_nodes(xyz)
+    eng.knn(84)
+    idx, D = eng.stencils()
+    nn, deg = variable_stencil_sizes(D)
+    eng.configure(cfg)
+    eng.set_stencil_sizes(nn, deg)
+    report = eng.weights_autotune(sample=4096)
+    eng.weights()
+    stats = eng.weights_stats()
+    Dx, Dy, L, st = eng.get_weights()
+    yield dict(eng=eng, xyz=xyz, cfg=cfg, idx=idx, nn=nn, deg=deg, W=(Dx, Dy, L, st), report=report, stats=stats)
+    eng.close()
+
+
+def test_sizes_and_autotuner(var):
+    nn, deg = var["nn"], var["deg"]
+    assert nn.min() == 31 and nn.max() == 84 and set(np.unique(deg).tolist()) == {3, 4, 5}
+    rep = var["report"]
+    print("autotune (variant, ms, systems/s):", rep)
+    timed = [r for r in rep if r[1] > 0]
+    assert len(timed) >= 2
+    ms, systems, v = var["stats"]
+    assert v == min(timed, key=lambda r: r[1])[0]
+    assert systems == 2 * len(nn)
+    print(f"weights: {systems:.0f} systems in {ms:.1f} ms = {systems / (ms * 1e-3):.3e} solves/s (variant {v})")
+
+
+def test_uneven_weights_match_oracle(var):
+    Dx, Dy, L, st = var["W"]
+    assert not st.any()
+    oDx, oDy, oL, ost = O.weights_var(var["cfg"], var["xyz"], var["idx"], var["nn"], var["deg"])
+    assert not ost.any()
+    for a, b in ((Dx, oDx), (Dy, oDy)):
+        err = np.max(np.abs(a - b), axis=1) / np.max(np.abs(b), axis=1)
+        assert err.max() < 1e-9, err.max()
+    errL = np.max(np.abs(L - oL), axis=1) / np.max(np.abs(oL), axis=1)
+    assert errL.max() < 1e-5, errL.max()
+    for p in range(0, len(Dx), 997):
+        assert np.all(Dx[p, var["nn"][p]:] == 0) and np.all(L[p, var["nn"][p]:] == 0)
+
+
+def _saddle(xyz, idx_row, n, d, phs=7):
+    """The config-mode gradient system of one node, assembled in numpy: rotate to the node's
+    tangent frame (pole fixed), shift to the centre, scale by the stencil radius."""
+    H = O.householder(xyz[idx_row[0]], fix_pole=True)
+    X = (H @ xyz[idx_row[:n]].T).T
+    X = X - X[0]
+    s = np.max(np.linalg.norm(X, axis=1))
+    X = X / s
+    r = np.linalg.norm(X[:, None, :] - X[None, :, :], axis=2)
+    mons = [(p - q, q) for p in range(d + 1) for q in range(p + 1)]
+    P = np.stack([X[:, 0] ** a * X[:, 1] ** b for a, b in mons], axis=1)
+    M = len(mons)
+    A = np.zeros((n + M, n + M))
+    A[:n, :n] = r ** phs
+    A[:n, n:] = P
+    A[n:, :n] = P.T
+    r0 = np.linalg.norm(X, axis=1)
+    b = np.zeros((n + M, 2))
+    b[:n, 0] = phs * (-X[:, 0]) * r0 ** (phs - 2)
+    b[:n, 1] = phs * (-X[:, 1]) * r0 ** (phs - 2)
+    b[n + mons.index((1, 0)), 0] = 1.0
+    b[n + mons.index((0, 1)), 1] = 1.0
+    return A, b, s
+
+
+def test_uneven_weights_against_lapack(var):
+    """Sampled nodes of every degree: the GPU weights against numpy.linalg.solve (LAPACK dgesv)
+    of the same saddle system, and the GPU solution's residual in that system."""
+    Dx, Dy, L, st = var["W"]
+    worst_w, worst_r = 0.0, 0.0
+    for p in list(range(0, len(Dx), 641)):
+        n, d = int(var["nn"][p]), int(var["deg"][p])
+        A, b, s = _saddle(var["xyz"], var["idx"][p], n, d)
+        ref = np.linalg.solve(A, b)
+        got = np.zeros_like(ref)
+        got[:n, 0] = Dx[p, :n] * s
+        got[:n, 1] = Dy[p, :n] * s
+        worst_w = max(worst_w, np.max(np.abs(got[:n] - ref[:n])) / np.max(np.abs(ref[:n])))
+        # residual of the weight part: complete the polynomial part by least squares
+        got[n:] = np.linalg.lstsq(A[:, n:], b - A[:, :n] @ got[:n], rcond=None)[0]
+        res = np.linalg.norm(A @ got - b) / (np.linalg.norm(A) * np.linalg.norm(got))
+        worst_r = max(worst_r, res)
+    print(f"vs LAPACK: max weight difference {worst_w:.2e}, max scaled residual {worst_r:.2e}")
+    assert worst_w < 1e-8
+    assert worst_r < 1e-12
+
+
+def test_time_loop_on_padded_rows(var):
+    eng, cfg = var["eng"], var["cfg"]
+    Dx, Dy, L, _ = var["W"]
+    q0 = O.initial(E.IC_BELLS2, cfg, var["xyz"], seed=1)
+    eng.set_state(q0)
+    dt = courant_dt(cfg, len(q0))
+    eng.run(F64, 10, dt)
+    q = eng.state(F64)
+    oq = O.Problem(cfg, var["xyz"], var["idx"], Dx, Dy, L).rk4(q0, 10, dt)
+    assert np.linalg.norm(q - oq) / np.linalg.norm(oq) < 1e-11
+    eng.run(F32, 10, dt)
+    assert np.linalg.norm(eng.state(F32) - oq) / np.linalg.norm(oq) < 1e-4
