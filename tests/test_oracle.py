@@ -420,3 +420,27 @@ def test_variable_stencil_weights(golden):
                 sc = np.abs(vDx[p, :n]) @ np.abs(v) + 1
                 assert abs(vDx[p, :n] @ v - (1.0 if (a, b) == (1, 0) else 0.0)) < 1e-9 * sc
                 assert abs(vDy[p, :n] @ v - (1.0 if (a, b) == (0, 1) else 0.0)) < 1e-9 * sc
+
+
+def test_gaussian_delta4_is_dissipative_and_phs_r9_is_not(golden):
+    """The assembled config-mode Delta^4 operator on 2562 icosahedral nodes: the Gaussian +
+    constant form (the default) has no eigenvalue of the anti-diffusive sign, so hv_coef < 0
+    damps every mode; the PHS r^9 + monomial form has hundreds, which is why it is not the
+    default."""
+    import math as _m
+    N = 2562
+    xyz = golden.icos(N)[:, :3]
+    idx, _ = O.knn(xyz, 55)
+    h8 = (_m.sqrt(8 * _m.pi / (_m.sqrt(3) * N))) ** 8
+    counts = {}
+    for hv in ("gauss", "phs"):
+        cfg = config_deform(N, Nv=10, n=55, deg=4, nv=5, hv=hv)
+        _, _, L, st = O.weights(cfg, xyz, idx)
+        assert not st.any()
+        M = np.zeros((N, N))
+        np.add.at(M, (np.repeat(np.arange(N), 55), idx.ravel()), L.ravel())
+        ev = np.linalg.eigvals(M) * h8
+        counts[hv] = int(np.sum(ev.real < -1e-8 * np.max(np.abs(ev))))
+        # a constant field is in the kernel: the rows sum to zero
+        assert np.max(np.abs(L.sum(axis=1))) * h8 < 1e-8 * np.max(np.abs(ev))
+    assert counts["gauss"] == 0 and counts["phs"] > 100, counts
