@@ -59,9 +59,14 @@ def test_uneven_weights_match_oracle(var):
     # condition numbers 1e13..1e18: two backward-stable LU solutions agree only to the
     # conditioning, so the Delta^4 weights are checked by their backward error in the system
     # (test_gaussian_delta4_backward_error) and here through the operator on smooth fields
+    # (nodes whose system is numerically singular have weights orders of magnitude above the
+    # median and are left to the backward-error test; at most 0.1% of the nodes)
     xyz, idx = var["xyz"], var["idx"]
+    mag = np.max(np.abs(oL), axis=1)
+    ok = mag < 100 * np.median(mag)
+    assert np.mean(ok) > 0.999, np.sum(~ok)
     for f in (xyz[:, 2] ** 2, xyz[:, 0] * xyz[:, 1]):
-        a, b = np.einsum("ij,ij->i", L, f[idx]), np.einsum("ij,ij->i", oL, f[idx])
+        a, b = np.einsum("ij,ij->i", L, f[idx])[ok], np.einsum("ij,ij->i", oL, f[idx])[ok]
         assert np.max(np.abs(a - b)) < 1e-4 * np.max(np.abs(b))
     for p in range(0, len(Dx), 997):
         assert np.all(Dx[p, var["nn"][p]:] == 0) and np.all(L[p, var["nn"][p]:] == 0)
