@@ -136,8 +136,9 @@ int serbf_weights(serbf_engine *e);
  * stride n (the largest size, from serbf_knn / serbf_set_stencils); node i uses its first
  * nn[i] neighbours (2 <= nn[i] <= n) and tangent-plane monomials of degree deg[i] (0..6, below
  * 2 hv_order), with nn[i] + (deg+1)(deg+2)/2 <= 112; its other row entries get zero weights, so
- * the time loop runs unchanged on the padded rows.  All the uneven systems are solved in one
- * persistent launch by serbf_weights.  nn = deg = NULL returns to uniform stencils.  Call after
+ * the time loop runs unchanged on the padded rows.  The Gaussian Delta^k shape parameter of a
+ * node grows as sqrt(n / nn[i]), keeping eps times the stencil radius the same for every size.
+ * All the uneven systems are solved in one persistent launch by serbf_weights.  nn = deg = NULL returns to uniform stencils.  Call after
  * serbf_configure. */
 int serbf_set_stencil_sizes(serbf_engine *e, const int *nn, const int *deg);
 /* Autotuner of the batched weight solves: times each schedule variant that fits (refinement

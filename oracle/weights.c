@@ -227,7 +227,7 @@ static double lap_k_phs_coef(int K, int k)
  * x^(p-i) y^i (p <= d).  Gradient weights are divided by s, Delta^k weights by s^2k. */
 static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, const double H[9],
                             double *wx, double *wy, double *wl, double *wl1, double *wl2,
-                            int *info2)
+                            int eps_nmax, int *info2)
 {
     const int d = cfg->poly_deg, M = (d + 1) * (d + 2) / 2, m = n + M;
     double *xs = malloc(sizeof(double) * 3 * n);
@@ -257,7 +257,10 @@ static int cfg_node_weights(const serbf_config *cfg, int n, const double *X, con
      * rbffd_DMs.c:188,223-228, L_garbf rbffd_DMs.c:314-328) with the shape parameter
      * hv_eps of the unit sphere, i.e. eps * s on the scaled stencil */
     const int gauss = kk > 0 && K == 0;
-    const double geps = cfg->hv_eps * s;
+    /* eps * s, and with per-node stencil sizes (n < cfg->n here only through orc_weights_var,
+     * which passes the row stride in eps_n) eps grows as sqrt(n_max / n) so that eps times the
+     * stencil radius is the same for every stencil size */
+    const double geps = cfg->hv_eps * s * (eps_nmax > n ? sqrt((double)eps_nmax / n) : 1.0);
     const double cL = kk > 0 ? lap_k_phs_coef(K, kk) : 0.0;
     for (int i = 0; i < n; i++) {
         for (int j = 0; j < n; j++) {
@@ -368,7 +371,7 @@ int orc_weights_lap(const serbf_config *cfg, int N, const double *x, const doubl
             orc_householder(x[i], y[i], z[i], 1, H);
             info1 = cfg_node_weights(cfg, n, X, H, Dx + (size_t)i * n, Dy + (size_t)i * n,
                                      L + (size_t)i * n, L1 ? L1 + (size_t)i * n : NULL,
-                                     L2 ? L2 + (size_t)i * n : NULL, &info2);
+                                     L2 ? L2 + (size_t)i * n : NULL, n, &info2);
         }
         status[i] = info1 ? 1 : (info2 ? 2 : 0);
         bad += status[i] != 0;
@@ -401,7 +404,7 @@ int orc_weights_var(const serbf_config *cfg, int N, const double *x, const doubl
         orc_householder(x[i], y[i], z[i], 1, H);
         double *dx = Dx + (size_t)i * stride, *dy = Dy + (size_t)i * stride, *dl = L + (size_t)i * stride;
         int info2 = 0;
-        const int info1 = cfg_node_weights(&c, n, X, H, dx, dy, dl, NULL, NULL, &info2);
+        const int info1 = cfg_node_weights(&c, n, X, H, dx, dy, dl, NULL, NULL, stride, &info2);
         for (int s = n; s < stride; s++)
             dx[s] = dy[s] = dl[s] = 0.0;
         status[i] = info1 ? 1 : (info2 ? 2 : 0);

@@ -402,9 +402,11 @@ __global__ void __launch_bounds__(64) weights_kernel(WArgs a)
         wave_sync();
         // Delta^k system
         if (a.hv > 0) {
-            // the Gaussian shape parameter of the unit sphere, in the scaled solve frame
+            // the Gaussian shape parameter of the unit sphere, in the scaled solve frame; with
+            // per-node stencil sizes it grows as sqrt(n_max / n), so that eps times the stencil
+            // radius is the same for every size
             WArgs ag = a;
-            ag.eps = a.eps * scale;
+            ag.eps = a.eps * scale * (a.nn && n < stride ? sqrt((double)stride / n) : 1.0);
             assemble<MAXM, COPY>(A, X, B0, A0, n, m2, 1, 1, ag, lane);
             int bad = factor_solve<MAXM, COPY>(A, X, B0, XS, A0, piv, n, m2, 1, 1, ag, lane);
             if (bad && !st) st = 2;
