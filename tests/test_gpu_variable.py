@@ -132,7 +132,7 @@ def test_gaussian_delta4_backward_error(var):
         X = X - X[0]
         s = np.max(np.linalg.norm(X, axis=1))
         X = X / s
-        eps = cfg.hv_eps * s
+        eps = cfg.hv_eps * s * np.sqrt(cfg.n / n)   # the per-size scaling of the shape parameter
         r = np.linalg.norm(X[:, None, :] - X[None, :, :], axis=2)
         A = np.zeros((n + 1, n + 1))
         A[:n, :n] = np.exp(-(eps * r) ** 2)
