@@ -24,7 +24,8 @@ def _dbs(path):
 
 
 def _short(name):
-    return name.split("(")[0].replace("void ", "").replace("serbf::", "")
+    name = name.replace("(anonymous namespace)::", "").replace("void ", "").replace("serbf::", "")
+    return name.split("(")[0]
 
 
 def kernel_stats(path):
