@@ -183,7 +183,28 @@ __device__ void point_features(const FeatArgs<T> &a, int p, int tr, int v, float
     T gx = T(0), gy = T(0), l1 = T(0), l2 = T(0), l4 = T(0);
     const size_t row = (size_t)p * s.n;
     const T *qcol = s.qin + (size_t)tr * S + vv;
-    for (int k = 0; k < s.n; k++) {
+    // batches of 8 neighbours: the index and value loads of a batch are all in flight before
+    // the first use (the loop is memory-latency bound otherwise)
+    constexpr int UB = 8;
+    int k = 0;
+    for (; k + UB <= s.n; k += UB) {
+        int nb[UB];
+        T q[UB];
+#pragma unroll
+        for (int u = 0; u < UB; u++) nb[u] = s.idx[row + k + u];
+#pragma unroll
+        for (int u = 0; u < UB; u++) q[u] = qcol[(size_t)nb[u] * NT * S];
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const size_t o = row + k + u;
+            gx = fma(s.Wx[o], q[u], gx);
+            gy = fma(s.Wy[o], q[u], gy);
+            l1 = fma(a.WL1[o], q[u], l1);
+            l2 = fma(a.WL2[o], q[u], l2);
+            l4 = fma(s.WL[o], q[u], l4);
+        }
+    }
+    for (; k < s.n; k++) {
         const T q = qcol[(size_t)s.idx[row + k] * NT * S];
         gx = fma(s.Wx[row + k], q, gx);
         gy = fma(s.Wy[row + k], q, gy);
@@ -255,8 +276,6 @@ __global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T
     if ((*a.s.base + a.s.step_off) % every != 0) return;
     const long ncol = (long)a.s.N * a.ntr;
     const long ngroups = (ncol + CPW - 1) / CPW;   // 64-point groups, one per wavefront pass
-    FwdRegs r;
-    load_fwd(nv, lane, r);
     // grid-stride over the groups; the loop bound is wavefront-uniform
     for (long grp = (long)blockIdx.x * 4 + w; grp < ngroups; grp += (long)gridDim.x * 4) {
         long col = grp * CPW + lane / S;
@@ -274,6 +293,10 @@ __global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T
             xrow[k] = f2bf(x);
         }
         wave_sync();
+        // the network operands are (re)loaded after the features, so the two phases do not
+        // hold their registers at the same time (cache hits after the first pass)
+        FwdRegs r;
+        load_fwd(nv, lane, r);
 #pragma unroll 1
         for (int st = 0; st < 4; st++) {
             const v4s xb = ld4(&xs[w][(16 * st + c) * 16 + 4 * g]);
@@ -597,12 +620,12 @@ int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma, int every)
 {
     const long ncol = (long)e->N * e->ntr;
     const long groups = e->S == 16 ? (ncol + 3) / 4 : ncol;
-    // a fixed grid of 8 blocks (32 wavefronts) per CU striding over the 64-point groups
+    // a fixed grid of 16 blocks (64 wavefronts) per CU striding over the 64-point groups
     if (e->cus <= 0) {
         hipDeviceProp_t prop;
         e->cus = hipGetDeviceProperties(&prop, e->device) == hipSuccess ? prop.multiProcessorCount : 256;
     }
-    long blocks = (long)e->cus * 8;
+    long blocks = (long)e->cus * 16;
     if (blocks > (groups + 3) / 4) blocks = (groups + 3) / 4;
     const dim3 grid((unsigned)blocks);
     const NetView nv = view(e->closure);

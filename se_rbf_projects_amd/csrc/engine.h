@@ -157,6 +157,11 @@ int rk4_prepare(serbf_engine *e);
 // LDS-tiled stage kernel (tile.hip)
 int tile_plan(serbf_engine *e);
 bool tile_supported(const serbf_engine *e);
+// matrix-core stage kernel (mtile.hip)
+template <typename T>
+struct StageArgs;
+bool mtile_supported(const serbf_engine *e);
+int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<float> a);
 void rk4_free(serbf_engine *e);
 int rk4_rhs(serbf_engine *e, double t, double *F_host);
 int rk4_run(serbf_engine *e, int precision, int nsteps, double dt, double t0, bool timed,

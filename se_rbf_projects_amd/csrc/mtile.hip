@@ -1,0 +1,208 @@
+// Stage kernel on the matrix cores (gfx950): the horizontal stencil of a 16-node tile as a
+// dense product on v_mfma_f32_16x16x4f32.
+//
+// A work-group owns one tile of 16 consecutive (space-filling-curve ordered) nodes.  The
+// tile plan (tile.hip) gives the union of their stencils (U <= 256 nodes) and, for every
+// stencil entry, its position in the union.  The work-group
+//   1. scatters the combined advection weights u_x' Dx + u_y' Dy of its 16 nodes (and the
+//      Delta^k weights when the stage applies hyperviscosity) into dense 16 x U blocks in LDS,
+//   2. for each tracer, stages the union's 64-level columns in LDS (each gathered column is
+//      read once per tile instead of once per stencil entry),
+//   3. lets wavefront w compute levels 16w..16w+15 of the 16 nodes as W (16 x U) x Q (U x 16)
+//      on the matrix cores, then applies the vertical stencil (from the node's own staged
+//      column), the velocity and the fused RK4 update on its accumulator tile.
+// MFMA operand layout (16x16x4, lane l, c = l % 16, g = l / 16):
+//   A (16 x 4): A[c][g]     B (4 x 16): B[g][c]     D (16 x 16): D[4g + i][c], i = 0..3
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+#include "stage_common.h"
+
+namespace serbf {
+
+namespace {
+
+constexpr int kMTileNodes = 16;
+constexpr int kMTileUcap = 256;
+constexpr int kQLD = 68;        // staged column stride (64 levels + pad)
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma4(float a, float b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+struct MTilePlan {
+    const int *uni;              // [ntiles][ustride] global ids of the union
+    const int *ucount;           // [ntiles]
+    const unsigned short *lidx;  // [N][n] union position of each stencil entry
+    int ustride;
+};
+
+// LDS: sv[16][2] node velocities | W[16][WLD] | WL[16][WLD] (HV) | Q[UP][kQLD]
+__host__ __device__ inline int mtile_lds_floats(int up, bool hv)
+{
+    const int wld = up + 1;
+    return 32 + 16 * wld * (hv ? 2 : 1) + up * kQLD;
+}
+
+template <int STAGE, int HV>
+__global__ void __launch_bounds__(256) mtile_kernel(StageArgs<float> a, MTilePlan tp, int up)
+{
+    extern __shared__ float lds[];
+    constexpr int S = 64, NTR = 4;
+    const int wld = up + 1;
+    float *sv = lds;
+    float *W = lds + 32;
+    float *WLt = W + 16 * wld;
+    float *Q = W + 16 * wld * (HV ? 2 : 1);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int tile = xcd_remap(blockIdx.x, a.nblk);
+    const int p0 = tile * kMTileNodes;
+    const int U = tp.ucount[tile];
+    const int n = a.n;
+    const float *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
+
+    // ---- node velocities and the dense weight blocks ----
+    if (tid < kMTileNodes) {
+        const int p = p0 + tid < a.N ? p0 + tid : a.N - 1;
+        float ux, uy;
+        node_velocity<float>(a, a.basis + (size_t)p * kBasis, ts, ux, uy);
+        sv[2 * tid] = ux;
+        sv[2 * tid + 1] = uy;
+    }
+    for (int e = tid; e < 16 * wld * (HV ? 2 : 1); e += 256) W[e] = 0.f;
+    __syncthreads();
+    for (int e = tid; e < kMTileNodes * n; e += 256) {
+        const int t = e / n, s = e - t * n;
+        const int p = p0 + t;
+        if (p < a.N) {
+            const size_t o = (size_t)p * n + s;
+            const int k = tp.lidx[o];
+            W[t * wld + k] = sv[2 * t] * a.Wx[o] + sv[2 * t + 1] * a.Wy[o];
+            if (HV) WLt[t * wld + k] = a.WL[o];
+        }
+    }
+    const int *uni = tp.uni + (size_t)tile * tp.ustride;
+
+    for (int tr = 0; tr < NTR; tr++) {
+        // ---- stage the union's columns of tracer tr (rows U..up-1 are zero) ----
+        __syncthreads();   // weights written / previous tracer's reads done
+        for (int e = tid; e < up * 16; e += 256) {
+            const int k = e >> 4, j4 = (e & 15) * 4;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (k < U) v = *(const float4 *)(a.qin + ((size_t)uni[k] * NTR + tr) * S + j4);
+            float *d = Q + k * kQLD + j4;
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+        }
+        __syncthreads();
+
+        // ---- W (16 x up) x Q (up x 16 levels of wavefront w) ----
+        v4f acc = {0.f, 0.f, 0.f, 0.f}, accl = {0.f, 0.f, 0.f, 0.f};
+        const float *wa = W + c * wld + g;
+        const float *wl = WLt + c * wld + g;
+        const float *qb = Q + g * kQLD + 16 * w + c;
+        for (int k0 = 0; k0 < up; k0 += 4) {
+            const float b = qb[k0 * kQLD];
+            acc = mfma4(wa[k0], b, acc);
+            if (HV) accl = mfma4(wl[k0], b, accl);
+        }
+
+        // ---- vertical stencil, velocity, RK4 update: 4 nodes x 1 level per lane ----
+        const int v = 16 * w + c;
+        const float r_inv = a.lev[v];
+        const int vst = a.vstart[v < a.Nv ? v : 0];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int t = 4 * g + i;
+            const int p = p0 + t;
+            if (p >= a.N) continue;
+            // the node's own column: its staged row when stencil entry 0 is the node itself
+            // (kNN stencils), else global memory
+            const int ks = tp.lidx[(size_t)p * n];
+            const float *own = uni[ks] == p ? Q + ks * kQLD : a.qin + ((size_t)p * NTR + tr) * S;
+            float z = 0.f;
+            if (v < a.Nv)
+                for (int j = 0; j < a.nv; j++) {
+                    const int lv = vst + j;
+                    if (lv >= 0 && lv < a.Nv) z = fmaf(a.vW[v * kMaxNv + j], own[lv], z);
+                }
+            float upx, upy, upz;
+            level_velocity<float, S>(a, a.basis + (size_t)p * kBasis, ts, p, v, upx, upy, upz);
+            (void)upx;
+            (void)upy;
+            const size_t my = ((size_t)p * NTR + tr) * S + v;
+            float f = a.alpha * (acc[i] * r_inv + upz * z);
+            if (HV) f += (a.hv_mode ? a.gamma[my] : a.hv_coef) * accl[i];
+            const bool live = v < a.Nv;
+            if (!live) f = 0.f;
+            if (STAGE == 0) {
+                a.F[my] = f;
+            } else if (STAGE == 1) {
+                a.F[my] = f;
+                a.qout[my] = live ? own[v] + a.hdt * f : 0.f;
+            } else if (STAGE == 2 || STAGE == 3) {
+                const float h = STAGE == 2 ? a.hdt : a.dt;
+                a.F[my] = a.F[my] + 2.f * f;
+                a.qout[my] = live ? a.qT[my] + h * f : 0.f;
+            } else {
+                a.qout[my] = live ? a.qT[my] + a.dt6 * (a.F[my] + 1.f * f) : 0.f;
+            }
+        }
+    }
+}
+
+template <int STAGE, int HV>
+void launch_one(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp, int up, int bytes)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)mtile_kernel<STAGE, HV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((mtile_kernel<STAGE, HV>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
+}
+
+}  // namespace
+
+// The matrix-core stage kernel serves FP32, 4 tracers of 64 levels, flows whose horizontal
+// velocity does not depend on the level (config mode), and tile plans whose unions fit.
+bool mtile_supported(const serbf_engine *e)
+{
+    return e->ntr == 4 && e->S == 64 && e->cfg.mode == SERBF_MODE_CONFIG &&
+           (e->cfg.flow == SERBF_FLOW_DEFORM || e->cfg.flow == SERBF_FLOW_SOLID_BODY) && e->have_plan &&
+           e->tile_umax <= kMTileUcap;
+}
+
+// hv: 0 none, 1 hyperviscosity in this stage, 2 split (stage 1 only; not handled here)
+int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<float> a)
+{
+    SERBF_CHECK(hv != 2, SERBF_ERR_UNSUPPORTED, "matrix-core stage kernel: split hyperviscosity");
+    MTilePlan tp;
+    tp.uni = e->d_tuni;
+    tp.ucount = e->d_tcount;
+    tp.lidx = e->d_lidx;
+    tp.ustride = e->tile_ustride;
+    const int up = e->tile_umax;   // multiple of 4
+    const int bytes = mtile_lds_floats(up, hv != 0) * (int)sizeof(float);
+    a.nblk = e->ntiles;
+    switch (stage * 2 + (hv ? 1 : 0)) {
+    case 0: launch_one<0, 0>(e, a, tp, up, bytes); break;
+    case 1: launch_one<0, 1>(e, a, tp, up, bytes); break;
+    case 2: launch_one<1, 0>(e, a, tp, up, bytes); break;
+    case 3: launch_one<1, 1>(e, a, tp, up, bytes); break;
+    case 4: launch_one<2, 0>(e, a, tp, up, bytes); break;
+    case 5: launch_one<2, 1>(e, a, tp, up, bytes); break;
+    case 6: launch_one<3, 0>(e, a, tp, up, bytes); break;
+    case 7: launch_one<3, 1>(e, a, tp, up, bytes); break;
+    case 8: launch_one<4, 0>(e, a, tp, up, bytes); break;
+    default: launch_one<4, 1>(e, a, tp, up, bytes); break;
+    }
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
+}  // namespace serbf

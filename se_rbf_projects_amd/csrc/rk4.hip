@@ -23,6 +23,8 @@
 // contiguous range of nodes (its L2 sees the spatially local neighbour columns).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -700,6 +702,9 @@ void launch2(serbf_engine *e, int stage, int hv, bool xy, StageArgs<T> a)
 template <typename T>
 int launch_stage(serbf_engine *e, int stage, int hv, const StageArgs<T> &a)
 {
+    if constexpr (std::is_same<T, float>::value) {
+        if (e->kver == 4 && hv != 2 && mtile_supported(e)) return launch_mtile(e, stage, hv, a);
+    }
     if (e->kver == 3 && e->have_plan && !e->plan_failed && tile_supported(e)) {
         const bool xy = a.flow == VEL_TC2 || a.flow == VEL_ARRAY || (a.ref_mode && a.flow == VEL_DEFORM);
         launch_tile<T>(e, stage, hv, xy, a);
@@ -783,6 +788,7 @@ int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, doubl
     int rc = ensure_tscal<T>(e, sb, nsteps, dt, t0);
     if (rc) return rc;
     if (e->kver == 3 && tile_supported(e) && (rc = tile_plan(e))) return rc;
+    if (e->kver == 4 && e->ntr == 4 && e->S == 64 && e->n <= 128 && (rc = tile_plan(e))) return rc;
     const int zero = 0;
     SERBF_HIP(hipMemcpyAsync(e->d_base, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
     // the graph of kGraphSteps steps reads the step counter and bumps it at the end
