@@ -261,55 +261,170 @@ __global__ void __launch_bounds__(256) features_kernel(FeatArgs<T> a, float *fea
 }
 
 // closure coefficients: gamma[(p*ntr + tr)*S + v] = hv_coef * s, s = softplus(MLP(x)), with
-// s clamped to s_max, s = 1 where it is not finite; levels >= Nv get 0
+// s clamped to s_max, s = 1 where it is not finite; levels >= Nv get 0.
+//
+// A wavefront pass covers NPW = 64/S nodes (lane = node slot x level) and a chunk of up to
+// four tracers, i.e. 256 points.  The pass stages the nodes' stencil rows (indices and the
+// five weight sets) in the wavefront's LDS with one coalesced load, then gathers 8 neighbours
+// x 4 tracers per batch (32 independent loads in flight per lane), writes the normalised BF16
+// features of its 256 points to LDS and runs them through the network as 16 tiles of 16.
+template <typename T, int S>
+struct ApplyGeom {
+    static constexpr int NPW = 64 / S;
+    static constexpr int TRC = 4;                       // tracers per pass
+    static constexpr int PTS = 64 * TRC;                // points per pass
+    static constexpr size_t XS = (size_t)PTS * 16 * 2;  // BF16 features
+    static constexpr size_t OS = (size_t)PTS * 4;       // network outputs
+    __host__ __device__ static size_t wave_bytes(int npad) { return XS + OS + (size_t)NPW * npad * (5 * sizeof(T) + 4); }
+};
+
 template <typename T, int S>
 __global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, T hv_coef, float s_max,
-                                                    int every)
+                                                    int every, int npad)
 {
-    constexpr int CPW = 64 / S;
-    __shared__ alignas(16) unsigned short xs[4][64 * 16];
-    __shared__ float os[4][64];
+    using G = ApplyGeom<T, S>;
+    constexpr int NPW = G::NPW, TRC = G::TRC, UB = 8;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
     // re-evaluate only on steps that are multiples of `every` (the step counter lives on the
     // device, so one captured graph serves every cadence); other steps exit at once
     if ((*a.s.base + a.s.step_off) % every != 0) return;
-    const long ncol = (long)a.s.N * a.ntr;
-    const long ngroups = (ncol + CPW - 1) / CPW;   // 64-point groups, one per wavefront pass
-    // grid-stride over the groups; the loop bound is wavefront-uniform
+    const StageArgs<T> &s = a.s;
+    unsigned char *wb = lds_raw + (size_t)w * G::wave_bytes(npad);
+    unsigned short *xs = (unsigned short *)wb;
+    float *os = (float *)(wb + G::XS);
+    T *mw = (T *)(wb + G::XS + G::OS);            // [5][NPW * npad]: Wx, Wy, Delta, Delta^2, Delta^k
+    int *mi = (int *)(mw + 5 * NPW * npad);       // [NPW * npad]
+    const int rows = NPW * npad;
+    const int n = s.n, NT = a.ntr;
+    const int slot = lane / S, v = lane % S;
+    const int vv = v < s.Nv ? v : s.Nv - 1;
+    const int tchunks = (NT + TRC - 1) / TRC;
+    const long ngroups = ((long)s.N + NPW - 1) / NPW * tchunks;
+    const T *ts = s.tscal + ((size_t)(*s.base + s.step_off) * 4) * kTimeScalars;
+    const size_t cstride = (size_t)NT * S;
+    const T K = (T)kKappa;
+    // grid-stride over the passes; the loop bound is wavefront-uniform
     for (long grp = (long)blockIdx.x * 4 + w; grp < ngroups; grp += (long)gridDim.x * 4) {
-        long col = grp * CPW + lane / S;
-        const bool ok = col < ncol;
-        if (!ok) col = ncol - 1;
-        const int v = lane % S;
-        const int p = (int)(col / a.ntr), tr = (int)(col % a.ntr);
-        float f[kF];
-        T l4;
-        point_features<T, S>(a, p, tr, v, f, l4);
-        unsigned short *xrow = &xs[w][lane * 16];
+        const long ng = grp / tchunks;
+        const int tr0 = (int)(grp - ng * tchunks) * TRC;
+        const int node0 = (int)(ng * NPW);
+        for (int j = lane; j < rows; j += 64) {
+            const int sl = j / npad, k = j - sl * npad;
+            int p = node0 + sl;
+            if (p >= s.N) p = s.N - 1;
+            if (k < n) {
+                const size_t o = (size_t)p * n + k;
+                mi[j] = s.idx[o];
+                mw[j] = s.Wx[o];
+                mw[rows + j] = s.Wy[o];
+                mw[2 * rows + j] = a.WL1[o];
+                mw[3 * rows + j] = a.WL2[o];
+                mw[4 * rows + j] = s.WL[o];
+            } else {
+                mi[j] = p;
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const float x = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
-            xrow[k] = f2bf(x);
+                for (int m = 0; m < 5; m++) mw[m * rows + j] = T(0);
+            }
+        }
+        wave_sync();
+        int p = node0 + slot;
+        const bool pok = p < s.N;
+        if (!pok) p = s.N - 1;
+        int trc[TRC];
+#pragma unroll
+        for (int t = 0; t < TRC; t++) trc[t] = tr0 + t < NT ? tr0 + t : NT - 1;
+        T gx[TRC], gy[TRC], l1[TRC], l2[TRC], l4[TRC];
+#pragma unroll
+        for (int t = 0; t < TRC; t++) gx[t] = gy[t] = l1[t] = l2[t] = l4[t] = T(0);
+        const int *ri = mi + slot * npad;
+        const T *rw = mw + slot * npad;
+        const T *qv = s.qin + vv;
+        for (int k0 = 0; k0 < n; k0 += UB) {
+            int nb[UB];
+            T q[UB][TRC];
+#pragma unroll
+            for (int u = 0; u < UB; u++) nb[u] = ri[k0 + u];
+#pragma unroll
+            for (int u = 0; u < UB; u++)
+#pragma unroll
+                for (int t = 0; t < TRC; t++) q[u][t] = qv[(size_t)nb[u] * cstride + trc[t] * S];
+#pragma unroll
+            for (int u = 0; u < UB; u++) {
+                const T wx = rw[k0 + u], wy = rw[rows + k0 + u], w1 = rw[2 * rows + k0 + u],
+                        w2 = rw[3 * rows + k0 + u], wk = rw[4 * rows + k0 + u];
+#pragma unroll
+                for (int t = 0; t < TRC; t++) {
+                    gx[t] = fma(wx, q[u][t], gx[t]);
+                    gy[t] = fma(wy, q[u][t], gy[t]);
+                    l1[t] = fma(w1, q[u][t], l1[t]);
+                    l2[t] = fma(w2, q[u][t], l2[t]);
+                    l4[t] = fma(wk, q[u][t], l4[t]);
+                }
+            }
+        }
+        // tracer-independent features
+        T ux, uy, uz;
+        level_velocity<T, S>(s, s.basis + (size_t)p * kBasis, ts, p, vv, ux, uy, uz);
+        const T speed = sqrt(ux * ux + uy * uy + uz * uz);
+        const T h = (T)a.hloc[p];
+        const T h2 = h * h, h4 = h2 * h2, h8 = h4 * h4;
+        float fc[4];
+        fc[0] = (float)(h / (T)a.hbar);
+        fc[1] = (float)(speed * s.dt / ((T)a.R * h));
+        fc[2] = (float)(speed / (T)a.umax);
+        fc[3] = (float)((T(vv) + T(0.5)) / T(s.Nv));
+        const int vs0 = s.vstart[vv];
+#pragma unroll
+        for (int t = 0; t < TRC; t++) {
+            const T *own = s.qin + ((size_t)p * NT + trc[t]) * S;
+            T dz = T(0);
+            for (int j = 0; j < s.nv; j++) {
+                const int k = vs0 + j;
+                if (k >= 0 && k < s.Nv) dz = fma(s.vW[vv * kMaxNv + j], own[k], dz);
+            }
+            float f[kF];
+            f[0] = (float)asinh(K * h * sqrt(gx[t] * gx[t] + gy[t] * gy[t]));
+            f[1] = (float)asinh(K * h2 * l1[t]);
+            f[2] = (float)asinh(K * h4 * l2[t]);
+            f[3] = (float)asinh(K * h8 * l4[t]);
+            f[4] = fc[0];
+            f[5] = fc[1];
+            f[6] = fc[2];
+            f[7] = fc[3];
+            f[8] = (float)own[vv];
+            f[9] = (float)asinh(K * (T)a.dh * fabs(dz));
+            unsigned short *xrow = xs + (t * 64 + lane) * 16;
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const float x = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
+                xrow[k] = f2bf(x);
+            }
         }
         wave_sync();
         // the network operands are (re)loaded after the features, so the two phases do not
         // hold their registers at the same time (cache hits after the first pass)
         FwdRegs r;
         load_fwd(nv, lane, r);
+        const int tiles = 4 * (NT - tr0 < TRC ? NT - tr0 : TRC);
 #pragma unroll 1
-        for (int st = 0; st < 4; st++) {
-            const v4s xb = ld4(&xs[w][(16 * st + c) * 16 + 4 * g]);
-            float h1[4][4], h2[4][4], o[4];
-            fwd_tile(r, xb, h1, h2, o);
-            if (c < 4) os[w][16 * st + 4 * g + c] = pick4(o, c);
+        for (int st = 0; st < tiles; st++) {
+            const v4s xb = ld4(xs + (16 * st + c) * 16 + 4 * g);
+            float h1[4][4], hh2[4][4], o[4];
+            fwd_tile(r, xb, h1, hh2, o);
+            if (c < 4) os[16 * st + 4 * g + c] = pick4(o, c);
         }
         wave_sync();
-        float sv = softplus_f(os[w][lane]);
-        if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
-        if (sv > s_max) sv = s_max;
-        if (ok) gamma[((size_t)p * a.ntr + tr) * S + v] = v < a.s.Nv ? hv_coef * (T)sv : T(0);
-        wave_sync();   // xs / os are rewritten by the next pass
+#pragma unroll
+        for (int t = 0; t < TRC; t++) {
+            if (tr0 + t >= NT || !pok) continue;
+            float sv = softplus_f(os[t * 64 + lane]);
+            if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
+            if (sv > s_max) sv = s_max;
+            gamma[((size_t)p * NT + tr0 + t) * S + v] = v < s.Nv ? hv_coef * (T)sv : T(0);
+        }
+        wave_sync();   // the LDS of the pass is rewritten by the next one
     }
 }
 
@@ -615,25 +730,45 @@ int ensure_hloc(serbf_engine *e)
     return SERBF_OK;
 }
 
+template <typename T, int S>
+void launch_apply_s(serbf_engine *e, dim3 grid, size_t bytes, const FeatArgs<T> &a, const NetView &nv, T *gamma,
+                    T hv, float smax, int every, int npad)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)apply_kernel<T, S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((apply_kernel<T, S>), grid, dim3(256), bytes, e->stream, a, nv, gamma, hv, smax, every, npad);
+}
+
 template <typename T>
 int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma, int every)
 {
-    const long ncol = (long)e->N * e->ntr;
-    const long groups = e->S == 16 ? (ncol + 3) / 4 : ncol;
-    // a fixed grid of 16 blocks (64 wavefronts) per CU striding over the 64-point groups
+    const int npad = (e->n + 7) / 8 * 8;
+    const int npw = 64 / e->S;
+    const long passes = ((long)e->N + npw - 1) / npw * ((e->ntr + 3) / 4);
+    // a fixed grid of 8 blocks (32 wavefronts) per CU striding over the passes
     if (e->cus <= 0) {
         hipDeviceProp_t prop;
         e->cus = hipGetDeviceProperties(&prop, e->device) == hipSuccess ? prop.multiProcessorCount : 256;
     }
-    long blocks = (long)e->cus * 16;
-    if (blocks > (groups + 3) / 4) blocks = (groups + 3) / 4;
+    long blocks = (long)e->cus * 8;
+    if (blocks > (passes + 3) / 4) blocks = (passes + 3) / 4;
     const dim3 grid((unsigned)blocks);
     const NetView nv = view(e->closure);
     const T hv = (T)e->cfg.hv_coef;
     const float smax = (float)e->closure_smax;
-    if (e->S == 16)
-        hipLaunchKernelGGL((apply_kernel<T, 16>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax, every);
-    else hipLaunchKernelGGL((apply_kernel<T, 64>), grid, dim3(256), 0, e->stream, a, nv, gamma, hv, smax, every);
+    if (e->S == 16) {
+        const size_t bytes = 4 * ApplyGeom<T, 16>::wave_bytes(npad);
+        SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "closure: stencil too large for the LDS");
+        launch_apply_s<T, 16>(e, grid, bytes, a, nv, gamma, hv, smax, every, npad);
+    } else {
+        const size_t bytes = 4 * ApplyGeom<T, 64>::wave_bytes(npad);
+        SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "closure: stencil too large for the LDS");
+        launch_apply_s<T, 64>(e, grid, bytes, a, nv, gamma, hv, smax, every, npad);
+    }
     SERBF_HIP(hipGetLastError());
     return SERBF_OK;
 }

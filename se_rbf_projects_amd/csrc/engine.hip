@@ -503,7 +503,7 @@ int serbf_set_hv_split(serbf_engine *e, int split)
 
 int serbf_set_kernel_version(serbf_engine *e, int version)
 {
-    SERBF_CHECK(e && version >= 1 && version <= 4, SERBF_ERR_ARG, "set_kernel_version: 1, 2, 3 or 4");
+    SERBF_CHECK(e && version >= 1 && version <= 5, SERBF_ERR_ARG, "set_kernel_version: 1 to 5");
     e->kver = version;
     return SERBF_OK;
 }

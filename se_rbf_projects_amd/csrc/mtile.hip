@@ -15,6 +15,8 @@
 //   A (16 x 4): A[c][g]     B (4 x 16): B[g][c]     D (16 x 16): D[4g + i][c], i = 0..3
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "engine.h"
 #include "stage_common.h"
 
@@ -166,6 +168,185 @@ void launch_one(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp,
     hipLaunchKernelGGL((mtile_kernel<STAGE, HV>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
 }
 
+// ---------------------------------------------------------------------------
+// Version 5: the same 16-node tiles and dense weight blocks, with the state read straight
+// from global memory as the MFMA B operand.
+//
+// Lane (c, g) of wavefront w loads level 16w + c of union column k0 + g: the four wavefronts
+// together read each union column once per tile (the stencil-by-stencil gather of the column
+// kernel reads it once per stencil entry, about 5.5 times as often).  Nothing of the state is
+// staged in LDS, so the main loop has no barrier; LDS holds only the dense weight blocks and the
+// union offsets (about 34 KB at 256 union columns), which leaves room for four or five
+// work-groups per CU.  Each step of 16 union columns issues 16 independent loads (4 column
+// groups x 4 tracers) and 32 MFMAs into 8 independent accumulators (4 tracers x advection and
+// Delta^k).  After the loop the LDS is reused for the tile's own columns (zero-padded by 8
+// levels on either side) for the vertical stencil and the RK4 update.
+constexpr int kOwnLD = 80;   // own-column stride: 8 + 64 + 8 levels
+
+__host__ __device__ inline int mtile2_lds_floats(int up, bool hv)
+{
+    const int wld = up + 1;
+    const int wreg = 16 * wld * (hv ? 2 : 1);
+    const int oreg = kMTileNodes * 4 * kOwnLD;
+    return 32 + (wreg > oreg ? wreg : oreg) + up;
+}
+
+template <int STAGE, int HV>
+__global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<float> a, MTilePlan tp, int up)
+{
+    extern __shared__ float lds[];
+    constexpr int S = 64, NTR = 4;
+    const int wld = up + 1;
+    const int wreg = 16 * wld * (HV ? 2 : 1), oreg = kMTileNodes * NTR * kOwnLD;
+    float *sv = lds;
+    float *W = lds + 32;
+    float *WLt = W + 16 * wld;
+    float *own = W;                                   // reused after the main loop
+    int *uoff = (int *)(W + (wreg > oreg ? wreg : oreg));
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int tile = xcd_remap(blockIdx.x, a.nblk);
+    const int p0 = tile * kMTileNodes;
+    const int U = tp.ucount[tile];
+    const int u16 = (U + 15) & ~15;
+    const int n = a.n;
+    const float *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
+    const int *uni = tp.uni + (size_t)tile * tp.ustride;
+
+    // ---- node velocities, union offsets, dense weight blocks ----
+    if (tid < kMTileNodes) {
+        const int p = p0 + tid < a.N ? p0 + tid : a.N - 1;
+        float ux, uy;
+        node_velocity<float>(a, a.basis + (size_t)p * kBasis, ts, ux, uy);
+        sv[2 * tid] = ux;
+        sv[2 * tid + 1] = uy;
+    }
+    for (int k = tid; k < u16; k += 256) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
+    for (int e = tid; e < wreg; e += 256) W[e] = 0.f;
+    __syncthreads();
+    for (int e = tid; e < kMTileNodes * n; e += 256) {
+        const int t = e / n, s = e - t * n;
+        const int p = p0 + t;
+        if (p < a.N) {
+            const size_t o = (size_t)p * n + s;
+            const int k = tp.lidx[o];
+            W[t * wld + k] = sv[2 * t] * a.Wx[o] + sv[2 * t + 1] * a.Wy[o];
+            if (HV) WLt[t * wld + k] = a.WL[o];
+        }
+    }
+    __syncthreads();
+
+    // ---- D (16 nodes x 16 levels of wavefront w) = W (16 x u16) x Q (u16 x 16), per tracer ----
+    v4f acc[NTR], accl[NTR];
+#pragma unroll
+    for (int t = 0; t < NTR; t++) {
+        acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+        accl[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    const float *qw = a.qin + 16 * w + c;
+    const float *wa = W + c * wld + g;
+    const float *wl = WLt + c * wld + g;
+    for (int k0 = 0; k0 < u16; k0 += 16) {
+        int off[4];
+        float b[4][NTR], av[4], al[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) off[j] = uoff[k0 + 4 * j + g];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int t = 0; t < NTR; t++) b[j][t] = qw[off[j] + t * S];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            av[j] = wa[k0 + 4 * j];
+            if (HV) al[j] = wl[k0 + 4 * j];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int t = 0; t < NTR; t++) {
+                acc[t] = mfma4(av[j], b[j][t], acc[t]);
+                if (HV) accl[t] = mfma4(al[j], b[j][t], accl[t]);
+            }
+    }
+
+    // ---- own columns of the tile in LDS, 8 zero levels on either side ----
+    __syncthreads();   // every wavefront is done with W and uoff
+    for (int e = tid; e < kMTileNodes * NTR * (S / 4); e += 256) {
+        const int col = e >> 4, l4 = (e & 15) * 4;
+        const int t = col / NTR, tr = col - t * NTR;
+        const int p = p0 + t < a.N ? p0 + t : a.N - 1;
+        const float4 x = *(const float4 *)(a.qin + ((size_t)p * NTR + tr) * S + l4);
+        float *d = own + col * kOwnLD + 8 + l4;
+        d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w;
+    }
+    for (int e = tid; e < kMTileNodes * NTR * 16; e += 256) {
+        const int col = e >> 4, k = e & 15;
+        own[col * kOwnLD + (k < 8 ? k : S + k)] = 0.f;
+    }
+    __syncthreads();
+
+    // ---- vertical stencil, velocity, RK4 update: 4 nodes x 4 tracers x 1 level per lane ----
+    const int v = 16 * w + c;
+    const bool live = v < a.Nv;
+    const float r_inv = a.lev[v];
+    const int vst = a.vstart[live ? v : 0];
+    float z[4][NTR];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int t = 0; t < NTR; t++) z[i][t] = 0.f;
+    if (live)
+        for (int j = 0; j < a.nv; j++) {
+            const float wv = a.vW[v * kMaxNv + j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int t = 0; t < NTR; t++)
+                    z[i][t] = fmaf(wv, own[((4 * g + i) * NTR + t) * kOwnLD + 8 + vst + j], z[i][t]);
+        }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int nd = 4 * g + i;
+        const int p = p0 + nd;
+        if (p >= a.N) continue;
+        float upx, upy, upz;
+        level_velocity<float, S>(a, a.basis + (size_t)p * kBasis, ts, p, v, upx, upy, upz);
+        (void)upx;
+        (void)upy;
+#pragma unroll
+        for (int t = 0; t < NTR; t++) {
+            const size_t my = ((size_t)p * NTR + t) * S + v;
+            float f = a.alpha * (acc[t][i] * r_inv + upz * z[i][t]);
+            if (HV) f += (a.hv_mode ? a.gamma[my] : a.hv_coef) * accl[t][i];
+            if (!live) f = 0.f;
+            if (STAGE == 0) {
+                a.F[my] = f;
+            } else if (STAGE == 1) {
+                a.F[my] = f;
+                a.qout[my] = live ? own[(nd * NTR + t) * kOwnLD + 8 + v] + a.hdt * f : 0.f;
+            } else if (STAGE == 2 || STAGE == 3) {
+                const float h = STAGE == 2 ? a.hdt : a.dt;
+                a.F[my] = a.F[my] + 2.f * f;
+                a.qout[my] = live ? a.qT[my] + h * f : 0.f;
+            } else {
+                a.qout[my] = live ? a.qT[my] + a.dt6 * (a.F[my] + 1.f * f) : 0.f;
+            }
+        }
+    }
+}
+
+template <int STAGE, int HV>
+void launch_two(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp, int up, int bytes)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)mtile2_kernel<STAGE, HV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((mtile2_kernel<STAGE, HV>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
+}
+
 }  // namespace
 
 // The matrix-core stage kernel serves FP32, 4 tracers of 64 levels, flows whose horizontal
@@ -174,7 +355,7 @@ bool mtile_supported(const serbf_engine *e)
 {
     return e->ntr == 4 && e->S == 64 && e->cfg.mode == SERBF_MODE_CONFIG &&
            (e->cfg.flow == SERBF_FLOW_DEFORM || e->cfg.flow == SERBF_FLOW_SOLID_BODY) && e->have_plan &&
-           e->tile_umax <= kMTileUcap;
+           e->tile_umax <= kMTileUcap && (size_t)e->N * e->ntr * e->S < (size_t)INT_MAX;
 }
 
 // hv: 0 none, 1 hyperviscosity in this stage, 2 split (stage 1 only; not handled here)
@@ -186,9 +367,27 @@ int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<float> a)
     tp.ucount = e->d_tcount;
     tp.lidx = e->d_lidx;
     tp.ustride = e->tile_ustride;
+    a.nblk = e->ntiles;
+    if (e->kver == 5) {
+        const int up = (e->tile_umax + 15) & ~15;
+        const int bytes = mtile2_lds_floats(up, hv != 0) * (int)sizeof(float);
+        switch (stage * 2 + (hv ? 1 : 0)) {
+        case 0: launch_two<0, 0>(e, a, tp, up, bytes); break;
+        case 1: launch_two<0, 1>(e, a, tp, up, bytes); break;
+        case 2: launch_two<1, 0>(e, a, tp, up, bytes); break;
+        case 3: launch_two<1, 1>(e, a, tp, up, bytes); break;
+        case 4: launch_two<2, 0>(e, a, tp, up, bytes); break;
+        case 5: launch_two<2, 1>(e, a, tp, up, bytes); break;
+        case 6: launch_two<3, 0>(e, a, tp, up, bytes); break;
+        case 7: launch_two<3, 1>(e, a, tp, up, bytes); break;
+        case 8: launch_two<4, 0>(e, a, tp, up, bytes); break;
+        default: launch_two<4, 1>(e, a, tp, up, bytes); break;
+        }
+        SERBF_HIP(hipGetLastError());
+        return SERBF_OK;
+    }
     const int up = e->tile_umax;   // multiple of 4
     const int bytes = mtile_lds_floats(up, hv != 0) * (int)sizeof(float);
-    a.nblk = e->ntiles;
     switch (stage * 2 + (hv ? 1 : 0)) {
     case 0: launch_one<0, 0>(e, a, tp, up, bytes); break;
     case 1: launch_one<0, 1>(e, a, tp, up, bytes); break;

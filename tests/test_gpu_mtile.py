@@ -1,6 +1,8 @@
-"""The matrix-core stage kernel (version 4: 16-node tiles, dense W x Q products on
-v_mfma_f32_16x16x4f32) against the column-vector kernel and the FP64 path, with and without
-the learned closure's per-point coefficients."""
+"""The matrix-core stage kernels (16-node tiles, dense W x Q products on
+v_mfma_f32_16x16x4f32; version 4 stages the state in LDS, version 5 reads it directly as the
+matrix operand) against the column-vector kernel and the FP64 path, with and without
+hyperviscosity and with the learned closure's per-point coefficients.  10242 nodes leave a
+partial last tile."""
 import numpy as np
 import pytest
 
@@ -36,14 +38,15 @@ def _run(eng, q0, prec, kver, steps, dt):
     return eng.state(prec)
 
 
-def test_matrix_core_stage_matches_column_kernel(deform60):
+@pytest.mark.parametrize("kver", [4, 5])
+def test_matrix_core_stage_matches_column_kernel(deform60, kver):
     eng, q0, dt = deform60["eng"], deform60["q0"], deform60["dt"]
     q64 = _run(eng, q0, F64, 2, 20, dt)
     q2 = _run(eng, q0, F32, 2, 20, dt)
-    q4 = _run(eng, q0, F32, 4, 20, dt)
+    q4 = _run(eng, q0, F32, kver, 20, dt)
     d24 = np.linalg.norm(q4 - q2) / np.linalg.norm(q2)
     d64 = np.linalg.norm(q4 - q64) / np.linalg.norm(q64)
-    print(f"kver 4 vs kver 2 (FP32): {d24:.2e}; vs FP64: {d64:.2e}; kver 2 vs FP64: "
+    print(f"kver {kver} vs kver 2 (FP32): {d24:.2e}; vs FP64: {d64:.2e}; kver 2 vs FP64: "
           f"{np.linalg.norm(q2 - q64) / np.linalg.norm(q64):.2e}")
     assert d24 < 1e-5
     assert d64 < 1e-4
@@ -51,7 +54,27 @@ def test_matrix_core_stage_matches_column_kernel(deform60):
     assert np.max(np.abs(q4[:, 3] - 1.0)) < 1e-5
 
 
-def test_matrix_core_stage_with_closure(deform60):
+@pytest.mark.parametrize("kver", [4, 5])
+def test_matrix_core_stage_without_hyperviscosity(deform60, kver):
+    eng, cfg, q0, dt = deform60["eng"], deform60["cfg"], deform60["q0"], deform60["dt"]
+    coef = cfg.hv_coef
+    cfg.hv_coef = 0.0
+    eng.configure(cfg)
+    eng.weights()
+    try:
+        q64 = _run(eng, q0, F64, 2, 10, dt)
+        q2 = _run(eng, q0, F32, 2, 10, dt)
+        q4 = _run(eng, q0, F32, kver, 10, dt)
+    finally:
+        cfg.hv_coef = coef
+        eng.configure(cfg)
+        eng.weights()
+    assert np.linalg.norm(q4 - q2) / np.linalg.norm(q2) < 1e-5
+    assert np.linalg.norm(q4 - q64) / np.linalg.norm(q64) < 1e-4
+
+
+@pytest.mark.parametrize("kver", [4, 5])
+def test_matrix_core_stage_with_closure(deform60, kver):
     eng, q0, dt = deform60["eng"], deform60["q0"], deform60["dt"]
     rng = np.random.default_rng(4)
     params = CL.init_params(4, out_scale=0.5)
@@ -61,7 +84,7 @@ def test_matrix_core_stage_with_closure(deform60):
     eng.set_state(q0)
     eng.attach_closure(cl, every=4, s_max=16.0)
     q2 = _run(eng, q0, F32, 2, 12, dt)
-    q4 = _run(eng, q0, F32, 4, 12, dt)
+    q4 = _run(eng, q0, F32, kver, 12, dt)
     eng.attach_closure(None)
     cl.close()
     assert np.linalg.norm(q4 - q2) / np.linalg.norm(q2) < 1e-5
