@@ -173,11 +173,12 @@ int serbf_set_hv_split(serbf_engine *e, int split);
 /* Per-point hyperviscosity coefficients c[N*ntracers*Nv] (1/s) replacing hv_coef, e.g.
  * from the learned closure; NULL returns to the constant hv_coef. */
 int serbf_set_gamma(serbf_engine *e, const double *gamma);
-/* Stage kernel variant: 2 (default) column-vector form, 3 LDS-tiled (columns of 64
- * levels, 1 or 4 tracers; other shapes use 2), 4 matrix-core tiles (FP32, 4 tracers of 64
- * levels, config-mode solid-body or deformational flow, hyperviscosity in every stage; other
- * cases use 2), 5 matrix-core tiles with the state read directly as the matrix operand (same
- * cases as 4), 1 lane-per-level form. */
+/* Stage kernel variant: 5 (default) matrix-core tiles with the state read directly as the
+ * matrix operand (FP32 and FP64, 4 tracers of 64 levels, config-mode solid-body or
+ * deformational flow, hyperviscosity in every stage; other cases use 2), 2 column-vector
+ * form, 3 LDS-tiled (columns of 64 levels, 1 or 4 tracers; other shapes use 2), 4
+ * matrix-core tiles with the state staged in LDS (FP32, the cases of 5), 1 lane-per-level
+ * form. */
 int serbf_set_kernel_version(serbf_engine *e, int version);
 /* Neighbour columns each wavefront loads before consuming them (4, 8 or 16): the
  * memory-level parallelism / register trade-off of the stage kernel. */

@@ -1041,9 +1041,7 @@ int serbf_closure_gamma(serbf_engine *e, int precision, double t, double dt, dou
         const FeatArgs<double> a = feat_args<double>(e, e->s64, e->s64.qT, dt, d_ts, e->d_base + 1, 0);
         rc = launch_apply<double>(e, a, e->s64.gamma, 1);
         if (!rc) {
-            SERBF_HIP(hipMemcpy2DAsync(gamma, Nv * sizeof(double), e->s64.gamma, S * sizeof(double),
-                                       Nv * sizeof(double), cols, hipMemcpyDeviceToHost, e->stream));
-            SERBF_HIP(hipStreamSynchronize(e->stream));
+            rc = download_padded(e, gamma, e->s64.gamma, cols, Nv, S);
         }
         (void)hipFree(d_ts);
     }

@@ -118,7 +118,7 @@ struct serbf_engine {
     // hyperviscosity once per step (1) or in every stage (0)
     int hv_split = 0;
     // stage kernel: 3 LDS-tiled, 2 column-vector form, 1 lane-per-level form
-    int kver = 2;
+    int kver = 5;
     // neighbour loads issued together per wavefront in the stage kernel (4, 8, 16)
     int unroll = 8;
     serbf::GraphCache graph;
@@ -157,11 +157,18 @@ int rk4_prepare(serbf_engine *e);
 // LDS-tiled stage kernel (tile.hip)
 int tile_plan(serbf_engine *e);
 bool tile_supported(const serbf_engine *e);
+// Copies between the host layout [cols][Nv] and the device layout [cols][S] (levels padded to
+// S with zeros), each as one contiguous transfer through a host staging buffer (a strided 2-D
+// copy to or from pageable memory moves one row at a time).  Both return after the transfer.
+int upload_padded(serbf_engine *e, double *dst, const double *src, size_t cols, int Nv, int S);
+int download_padded(serbf_engine *e, double *dst, const double *src, size_t cols, int Nv, int S);
+
 // matrix-core stage kernel (mtile.hip)
 template <typename T>
 struct StageArgs;
 bool mtile_supported(const serbf_engine *e);
-int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<float> a);
+template <typename T>
+int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<T> a);
 void rk4_free(serbf_engine *e);
 int rk4_rhs(serbf_engine *e, double t, double *F_host);
 int rk4_run(serbf_engine *e, int precision, int nsteps, double dt, double t0, bool timed,

@@ -67,6 +67,38 @@ __global__ void stream_copy_kernel(size_t n16, const float4 *src, float4 *dst)
 
 using namespace serbf;
 
+namespace serbf {
+
+int upload_padded(serbf_engine *e, double *dst, const double *src, size_t cols, int Nv, int S)
+{
+    if (Nv == S) {
+        SERBF_HIP(hipMemcpyAsync(dst, src, cols * S * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        SERBF_HIP(hipStreamSynchronize(e->stream));
+        return SERBF_OK;
+    }
+    std::vector<double> tmp(cols * S, 0.0);
+    for (size_t c = 0; c < cols; c++) std::memcpy(&tmp[c * S], src + c * Nv, Nv * sizeof(double));
+    SERBF_HIP(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    return SERBF_OK;
+}
+
+int download_padded(serbf_engine *e, double *dst, const double *src, size_t cols, int Nv, int S)
+{
+    if (Nv == S) {
+        SERBF_HIP(hipMemcpyAsync(dst, src, cols * S * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        SERBF_HIP(hipStreamSynchronize(e->stream));
+        return SERBF_OK;
+    }
+    std::vector<double> tmp(cols * S);
+    SERBF_HIP(hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    for (size_t c = 0; c < cols; c++) std::memcpy(dst + c * Nv, &tmp[c * S], Nv * sizeof(double));
+    return SERBF_OK;
+}
+
+}  // namespace serbf
+
 extern "C" {
 
 const char *serbf_last_error(void) { return g_err.c_str(); }
@@ -408,8 +440,10 @@ int serbf_set_state(serbf_engine *e, const double *q)
     }
     const int Nv = e->cfg.Nv, S = e->S;
     const size_t cols = (size_t)e->N * e->ntr;
-    SERBF_HIP(hipMemcpy2DAsync(e->s64.qT, S * sizeof(double), q, Nv * sizeof(double), Nv * sizeof(double), cols,
-                               hipMemcpyHostToDevice, e->stream));
+    {
+        const int rc = upload_padded(e, e->s64.qT, q, cols, Nv, S);
+        if (rc) return rc;
+    }
     if (!e->d_q0) {
         SERBF_HIP(hipMalloc((void **)&e->d_q0, cols * S * sizeof(double)));
         SERBF_HIP(hipMemsetAsync(e->d_q0, 0, cols * S * sizeof(double), e->stream));
@@ -443,10 +477,7 @@ int serbf_get_state(serbf_engine *e, int precision, double *q)
             for (int v = 0; v < Nv; v++) q[c * Nv + v] = tmp[c * S + v];
         return SERBF_OK;
     }
-    SERBF_HIP(hipMemcpy2DAsync(q, Nv * sizeof(double), e->s64.qT, S * sizeof(double), Nv * sizeof(double), cols,
-                               hipMemcpyDeviceToHost, e->stream));
-    SERBF_HIP(hipStreamSynchronize(e->stream));
-    return SERBF_OK;
+    return download_padded(e, q, e->s64.qT, cols, Nv, S);
 }
 
 int serbf_rhs(serbf_engine *e, double t, double *F)
@@ -544,9 +575,9 @@ int serbf_set_gamma(serbf_engine *e, const double *gamma)
         SERBF_HIP(hipMalloc((void **)&e->s32.gamma, P * sizeof(float)));
         SERBF_HIP(hipMemsetAsync(e->s64.gamma, 0, P * sizeof(double), e->stream));
     }
-    SERBF_HIP(hipMemcpy2DAsync(e->s64.gamma, S * sizeof(double), gamma, Nv * sizeof(double), Nv * sizeof(double), cols,
-                               hipMemcpyHostToDevice, e->stream));
-    int rc = convert_to_f32(e, P, e->s64.gamma, e->s32.gamma);
+    int rc = upload_padded(e, e->s64.gamma, gamma, cols, Nv, S);
+    if (rc) return rc;
+    rc = convert_to_f32(e, P, e->s64.gamma, e->s32.gamma);
     if (rc) return rc;
     SERBF_HIP(hipStreamSynchronize(e->stream));
     e->hv_mode = 1;
@@ -564,8 +595,11 @@ int serbf_norms(serbf_engine *e, int precision, const double *q_ref, double *out
     if (q_ref) {
         SERBF_HIP(hipMalloc((void **)&tmp, cols * S * sizeof(double)));
         SERBF_HIP(hipMemsetAsync(tmp, 0, cols * S * sizeof(double), e->stream));
-        SERBF_HIP(hipMemcpy2DAsync(tmp, S * sizeof(double), q_ref, Nv * sizeof(double), Nv * sizeof(double), cols,
-                                   hipMemcpyHostToDevice, e->stream));
+        const int urc = upload_padded(e, tmp, q_ref, cols, Nv, S);
+        if (urc) {
+            (void)hipFree(tmp);
+            return urc;
+        }
         ref = tmp;
     }
     const int blocks = 1024;

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 
 #include "engine.h"
 #include "stage_common.h"
@@ -29,11 +30,31 @@ constexpr int kMTileUcap = 256;
 constexpr int kQLD = 68;        // staged column stride (64 levels + pad)
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ v4f mfma4(float a, float b, v4f c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+// v_mfma_f64_16x16x4f64: the same operand layout as the FP32 form; result value i of lane
+// (c, g) is D[g + 4i][c] (FP32: D[4g + i][c])
+__device__ __forceinline__ v4d mfma4(double a, double b, v4d c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <typename T>
+struct Acc4;
+template <>
+struct Acc4<float> {
+    typedef v4f type;
+    // tile row (node) of result value i of a lane in row group g
+    __device__ static int row(int g, int i) { return 4 * g + i; }
+};
+template <>
+struct Acc4<double> {
+    typedef v4d type;
+    __device__ static int row(int g, int i) { return g + 4 * i; }
+};
 
 struct MTilePlan {
     const int *uni;              // [ntiles][ustride] global ids of the union
@@ -169,8 +190,9 @@ void launch_one(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp,
 }
 
 // ---------------------------------------------------------------------------
-// Version 5: the same 16-node tiles and dense weight blocks, with the state read straight
-// from global memory as the MFMA B operand.
+// Version 5 (FP32 on v_mfma_f32_16x16x4f32, FP64 on v_mfma_f64_16x16x4f64): the same
+// 16-node tiles and dense weight blocks, with the state read straight from global memory as
+// the MFMA B operand.
 //
 // Lane (c, g) of wavefront w loads level 16w + c of union column k0 + g: the four wavefronts
 // together read each union column once per tile (the stencil-by-stencil gather of the column
@@ -183,25 +205,27 @@ void launch_one(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp,
 // levels on either side) for the vertical stencil and the RK4 update.
 constexpr int kOwnLD = 80;   // own-column stride: 8 + 64 + 8 levels
 
-__host__ __device__ inline int mtile2_lds_floats(int up, bool hv)
+template <typename T>
+__host__ __device__ inline int mtile2_lds_bytes(int up, bool hv)
 {
     const int wld = up + 1;
     const int wreg = 16 * wld * (hv ? 2 : 1);
     const int oreg = kMTileNodes * 4 * kOwnLD;
-    return 32 + (wreg > oreg ? wreg : oreg) + up;
+    return (32 + (wreg > oreg ? wreg : oreg)) * (int)sizeof(T) + up * (int)sizeof(int);
 }
 
-template <int STAGE, int HV>
-__global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<float> a, MTilePlan tp, int up)
+template <typename T, int STAGE, int HV>
+__global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan tp, int up)
 {
-    extern __shared__ float lds[];
+    extern __shared__ __align__(16) unsigned char m2_lds[];
+    typedef typename Acc4<T>::type VT;
     constexpr int S = 64, NTR = 4;
     const int wld = up + 1;
     const int wreg = 16 * wld * (HV ? 2 : 1), oreg = kMTileNodes * NTR * kOwnLD;
-    float *sv = lds;
-    float *W = lds + 32;
-    float *WLt = W + 16 * wld;
-    float *own = W;                                   // reused after the main loop
+    T *sv = (T *)m2_lds;
+    T *W = sv + 32;
+    T *WLt = W + 16 * wld;
+    T *own = W;                                       // reused after the main loop
     int *uoff = (int *)(W + (wreg > oreg ? wreg : oreg));
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int c = lane & 15, g = lane >> 4;
@@ -210,19 +234,19 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<float> a, MTilePl
     const int U = tp.ucount[tile];
     const int u16 = (U + 15) & ~15;
     const int n = a.n;
-    const float *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
+    const T *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
     const int *uni = tp.uni + (size_t)tile * tp.ustride;
 
     // ---- node velocities, union offsets, dense weight blocks ----
     if (tid < kMTileNodes) {
         const int p = p0 + tid < a.N ? p0 + tid : a.N - 1;
-        float ux, uy;
-        node_velocity<float>(a, a.basis + (size_t)p * kBasis, ts, ux, uy);
+        T ux, uy;
+        node_velocity<T>(a, a.basis + (size_t)p * kBasis, ts, ux, uy);
         sv[2 * tid] = ux;
         sv[2 * tid + 1] = uy;
     }
     for (int k = tid; k < u16; k += 256) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
-    for (int e = tid; e < wreg; e += 256) W[e] = 0.f;
+    for (int e = tid; e < wreg; e += 256) W[e] = T(0);
     __syncthreads();
     for (int e = tid; e < kMTileNodes * n; e += 256) {
         const int t = e / n, s = e - t * n;
@@ -237,18 +261,18 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<float> a, MTilePl
     __syncthreads();
 
     // ---- D (16 nodes x 16 levels of wavefront w) = W (16 x u16) x Q (u16 x 16), per tracer ----
-    v4f acc[NTR], accl[NTR];
+    VT acc[NTR], accl[NTR];
 #pragma unroll
     for (int t = 0; t < NTR; t++) {
-        acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-        accl[t] = v4f{0.f, 0.f, 0.f, 0.f};
+        acc[t] = VT{T(0), T(0), T(0), T(0)};
+        accl[t] = VT{T(0), T(0), T(0), T(0)};
     }
-    const float *qw = a.qin + 16 * w + c;
-    const float *wa = W + c * wld + g;
-    const float *wl = WLt + c * wld + g;
+    const T *qw = a.qin + 16 * w + c;
+    const T *wa = W + c * wld + g;
+    const T *wl = WLt + c * wld + g;
     for (int k0 = 0; k0 < u16; k0 += 16) {
         int off[4];
-        float b[4][NTR], av[4], al[4];
+        T b[4][NTR], av[4], al[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) off[j] = uoff[k0 + 4 * j + g];
 #pragma unroll
@@ -275,82 +299,105 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<float> a, MTilePl
         const int col = e >> 4, l4 = (e & 15) * 4;
         const int t = col / NTR, tr = col - t * NTR;
         const int p = p0 + t < a.N ? p0 + t : a.N - 1;
-        const float4 x = *(const float4 *)(a.qin + ((size_t)p * NTR + tr) * S + l4);
-        float *d = own + col * kOwnLD + 8 + l4;
-        d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w;
+        const Vec4<T> x = load4<T>(a.qin + ((size_t)p * NTR + tr) * S + l4);
+        T *d = own + col * kOwnLD + 8 + l4;
+        d[0] = x.v[0]; d[1] = x.v[1]; d[2] = x.v[2]; d[3] = x.v[3];
     }
     for (int e = tid; e < kMTileNodes * NTR * 16; e += 256) {
         const int col = e >> 4, k = e & 15;
-        own[col * kOwnLD + (k < 8 ? k : S + k)] = 0.f;
+        own[col * kOwnLD + (k < 8 ? k : S + k)] = T(0);
     }
     __syncthreads();
 
     // ---- vertical stencil, velocity, RK4 update: 4 nodes x 4 tracers x 1 level per lane ----
     const int v = 16 * w + c;
     const bool live = v < a.Nv;
-    const float r_inv = a.lev[v];
+    const T r_inv = a.lev[v];
     const int vst = a.vstart[live ? v : 0];
-    float z[4][NTR];
+    T z[4][NTR];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int t = 0; t < NTR; t++) z[i][t] = 0.f;
+        for (int t = 0; t < NTR; t++) z[i][t] = T(0);
     if (live)
         for (int j = 0; j < a.nv; j++) {
-            const float wv = a.vW[v * kMaxNv + j];
+            const T wv = a.vW[v * kMaxNv + j];
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int t = 0; t < NTR; t++)
-                    z[i][t] = fmaf(wv, own[((4 * g + i) * NTR + t) * kOwnLD + 8 + vst + j], z[i][t]);
+                    z[i][t] = fma(wv, own[(Acc4<T>::row(g, i) * NTR + t) * kOwnLD + 8 + vst + j], z[i][t]);
         }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const int nd = 4 * g + i;
+        const int nd = Acc4<T>::row(g, i);
         const int p = p0 + nd;
         if (p >= a.N) continue;
-        float upx, upy, upz;
-        level_velocity<float, S>(a, a.basis + (size_t)p * kBasis, ts, p, v, upx, upy, upz);
+        T upx, upy, upz;
+        level_velocity<T, S>(a, a.basis + (size_t)p * kBasis, ts, p, v, upx, upy, upz);
         (void)upx;
         (void)upy;
 #pragma unroll
         for (int t = 0; t < NTR; t++) {
             const size_t my = ((size_t)p * NTR + t) * S + v;
-            float f = a.alpha * (acc[t][i] * r_inv + upz * z[i][t]);
+            T f = a.alpha * (acc[t][i] * r_inv + upz * z[i][t]);
             if (HV) f += (a.hv_mode ? a.gamma[my] : a.hv_coef) * accl[t][i];
-            if (!live) f = 0.f;
+            if (!live) f = T(0);
             if (STAGE == 0) {
                 a.F[my] = f;
             } else if (STAGE == 1) {
                 a.F[my] = f;
-                a.qout[my] = live ? own[(nd * NTR + t) * kOwnLD + 8 + v] + a.hdt * f : 0.f;
+                a.qout[my] = live ? own[(nd * NTR + t) * kOwnLD + 8 + v] + a.hdt * f : T(0);
             } else if (STAGE == 2 || STAGE == 3) {
-                const float h = STAGE == 2 ? a.hdt : a.dt;
-                a.F[my] = a.F[my] + 2.f * f;
-                a.qout[my] = live ? a.qT[my] + h * f : 0.f;
+                const T h = STAGE == 2 ? a.hdt : a.dt;
+                a.F[my] = a.F[my] + T(2) * f;
+                a.qout[my] = live ? a.qT[my] + h * f : T(0);
             } else {
-                a.qout[my] = live ? a.qT[my] + a.dt6 * (a.F[my] + 1.f * f) : 0.f;
+                a.qout[my] = live ? a.qT[my] + a.dt6 * (a.F[my] + T(1) * f) : T(0);
             }
         }
     }
 }
 
-template <int STAGE, int HV>
-void launch_two(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp, int up, int bytes)
+template <typename T, int STAGE, int HV>
+void launch_two(serbf_engine *e, const StageArgs<T> &a, const MTilePlan &tp, int up, int bytes)
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)mtile2_kernel<STAGE, HV>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void *)mtile2_kernel<T, STAGE, HV>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((mtile2_kernel<STAGE, HV>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
+    hipLaunchKernelGGL((mtile2_kernel<T, STAGE, HV>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
+}
+
+template <typename T>
+int launch_mtile2(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTilePlan &tp)
+{
+    const int up = (e->tile_umax + 15) & ~15;
+    const int bytes = mtile2_lds_bytes<T>(up, hv != 0);
+    SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "matrix-core stage kernel: tile union too large");
+    switch (stage * 2 + (hv ? 1 : 0)) {
+    case 0: launch_two<T, 0, 0>(e, a, tp, up, bytes); break;
+    case 1: launch_two<T, 0, 1>(e, a, tp, up, bytes); break;
+    case 2: launch_two<T, 1, 0>(e, a, tp, up, bytes); break;
+    case 3: launch_two<T, 1, 1>(e, a, tp, up, bytes); break;
+    case 4: launch_two<T, 2, 0>(e, a, tp, up, bytes); break;
+    case 5: launch_two<T, 2, 1>(e, a, tp, up, bytes); break;
+    case 6: launch_two<T, 3, 0>(e, a, tp, up, bytes); break;
+    case 7: launch_two<T, 3, 1>(e, a, tp, up, bytes); break;
+    case 8: launch_two<T, 4, 0>(e, a, tp, up, bytes); break;
+    default: launch_two<T, 4, 1>(e, a, tp, up, bytes); break;
+    }
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
 }
 
 }  // namespace
 
-// The matrix-core stage kernel serves FP32, 4 tracers of 64 levels, flows whose horizontal
-// velocity does not depend on the level (config mode), and tile plans whose unions fit.
+// The matrix-core stage kernels serve 4 tracers of 64 levels, flows whose horizontal velocity
+// does not depend on the level (config mode), and tile plans whose unions fit (version 4: FP32;
+// version 5: FP32 and FP64).
 bool mtile_supported(const serbf_engine *e)
 {
     return e->ntr == 4 && e->S == 64 && e->cfg.mode == SERBF_MODE_CONFIG &&
@@ -359,7 +406,8 @@ bool mtile_supported(const serbf_engine *e)
 }
 
 // hv: 0 none, 1 hyperviscosity in this stage, 2 split (stage 1 only; not handled here)
-int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<float> a)
+template <typename T>
+int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<T> a)
 {
     SERBF_CHECK(hv != 2, SERBF_ERR_UNSUPPORTED, "matrix-core stage kernel: split hyperviscosity");
     MTilePlan tp;
@@ -368,40 +416,29 @@ int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<float> a)
     tp.lidx = e->d_lidx;
     tp.ustride = e->tile_ustride;
     a.nblk = e->ntiles;
-    if (e->kver == 5) {
-        const int up = (e->tile_umax + 15) & ~15;
-        const int bytes = mtile2_lds_floats(up, hv != 0) * (int)sizeof(float);
+    if constexpr (std::is_same<T, double>::value) {
+        return launch_mtile2<double>(e, stage, hv, a, tp);
+    } else {
+        if (e->kver == 5) return launch_mtile2<float>(e, stage, hv, a, tp);
+        const int up = e->tile_umax;   // multiple of 4
+        const int bytes = mtile_lds_floats(up, hv != 0) * (int)sizeof(float);
         switch (stage * 2 + (hv ? 1 : 0)) {
-        case 0: launch_two<0, 0>(e, a, tp, up, bytes); break;
-        case 1: launch_two<0, 1>(e, a, tp, up, bytes); break;
-        case 2: launch_two<1, 0>(e, a, tp, up, bytes); break;
-        case 3: launch_two<1, 1>(e, a, tp, up, bytes); break;
-        case 4: launch_two<2, 0>(e, a, tp, up, bytes); break;
-        case 5: launch_two<2, 1>(e, a, tp, up, bytes); break;
-        case 6: launch_two<3, 0>(e, a, tp, up, bytes); break;
-        case 7: launch_two<3, 1>(e, a, tp, up, bytes); break;
-        case 8: launch_two<4, 0>(e, a, tp, up, bytes); break;
-        default: launch_two<4, 1>(e, a, tp, up, bytes); break;
+        case 0: launch_one<0, 0>(e, a, tp, up, bytes); break;
+        case 1: launch_one<0, 1>(e, a, tp, up, bytes); break;
+        case 2: launch_one<1, 0>(e, a, tp, up, bytes); break;
+        case 3: launch_one<1, 1>(e, a, tp, up, bytes); break;
+        case 4: launch_one<2, 0>(e, a, tp, up, bytes); break;
+        case 5: launch_one<2, 1>(e, a, tp, up, bytes); break;
+        case 6: launch_one<3, 0>(e, a, tp, up, bytes); break;
+        case 7: launch_one<3, 1>(e, a, tp, up, bytes); break;
+        case 8: launch_one<4, 0>(e, a, tp, up, bytes); break;
+        default: launch_one<4, 1>(e, a, tp, up, bytes); break;
         }
         SERBF_HIP(hipGetLastError());
         return SERBF_OK;
     }
-    const int up = e->tile_umax;   // multiple of 4
-    const int bytes = mtile_lds_floats(up, hv != 0) * (int)sizeof(float);
-    switch (stage * 2 + (hv ? 1 : 0)) {
-    case 0: launch_one<0, 0>(e, a, tp, up, bytes); break;
-    case 1: launch_one<0, 1>(e, a, tp, up, bytes); break;
-    case 2: launch_one<1, 0>(e, a, tp, up, bytes); break;
-    case 3: launch_one<1, 1>(e, a, tp, up, bytes); break;
-    case 4: launch_one<2, 0>(e, a, tp, up, bytes); break;
-    case 5: launch_one<2, 1>(e, a, tp, up, bytes); break;
-    case 6: launch_one<3, 0>(e, a, tp, up, bytes); break;
-    case 7: launch_one<3, 1>(e, a, tp, up, bytes); break;
-    case 8: launch_one<4, 0>(e, a, tp, up, bytes); break;
-    default: launch_one<4, 1>(e, a, tp, up, bytes); break;
-    }
-    SERBF_HIP(hipGetLastError());
-    return SERBF_OK;
 }
+template int launch_mtile<float>(serbf_engine *, int, int, StageArgs<float>);
+template int launch_mtile<double>(serbf_engine *, int, int, StageArgs<double>);
 
 }  // namespace serbf

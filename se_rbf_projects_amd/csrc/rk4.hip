@@ -703,8 +703,9 @@ template <typename T>
 int launch_stage(serbf_engine *e, int stage, int hv, const StageArgs<T> &a)
 {
     if constexpr (std::is_same<T, float>::value) {
-        if ((e->kver == 4 || e->kver == 5) && hv != 2 && mtile_supported(e)) return launch_mtile(e, stage, hv, a);
+        if (e->kver == 4 && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
     }
+    if (e->kver == 5 && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
     if (e->kver == 3 && e->have_plan && !e->plan_failed && tile_supported(e)) {
         const bool xy = a.flow == VEL_TC2 || a.flow == VEL_ARRAY || (a.ref_mode && a.flow == VEL_DEFORM);
         launch_tile<T>(e, stage, hv, xy, a);
@@ -893,9 +894,7 @@ int rk4_rhs(serbf_engine *e, double t, double *F_host)
     int rc = launch_stage<double>(e, 0, hv_enabled(e) ? 1 : 0, a);
     if (!rc) {
         const size_t P = (size_t)e->N * e->ntr;
-        SERBF_HIP(hipMemcpy2DAsync(F_host, e->cfg.Nv * sizeof(double), e->s64.qA, e->S * sizeof(double),
-                                   e->cfg.Nv * sizeof(double), P, hipMemcpyDeviceToHost, e->stream));
-        SERBF_HIP(hipStreamSynchronize(e->stream));
+        rc = download_padded(e, F_host, e->s64.qA, P, e->cfg.Nv, e->S);
     }
     (void)hipFree(d_ts);
     return rc;
@@ -910,10 +909,12 @@ int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha
     double *d_u;
     SERBF_HIP(hipMalloc((void **)&d_u, (size_t)N * 3 * S * sizeof(double)));
     SERBF_HIP(hipMemsetAsync(d_u, 0, (size_t)N * 3 * S * sizeof(double), e->stream));
-    SERBF_HIP(hipMemcpy2DAsync(d_u, S * sizeof(double), Up_host, Nv * sizeof(double), Nv * sizeof(double),
-                               (size_t)N * 3, hipMemcpyHostToDevice, e->stream));
-    SERBF_HIP(hipMemcpy2DAsync(e->s64.qB, S * sizeof(double), q_host, Nv * sizeof(double), Nv * sizeof(double),
-                               (size_t)N, hipMemcpyHostToDevice, e->stream));
+    int urc = upload_padded(e, d_u, Up_host, (size_t)N * 3, Nv, S);
+    if (!urc) urc = upload_padded(e, e->s64.qB, q_host, (size_t)N, Nv, S);
+    if (urc) {
+        (void)hipFree(d_u);
+        return urc;
+    }
     double ts[4 * kTimeScalars] = {0};
     double *d_ts;
     SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
@@ -932,9 +933,7 @@ int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha
     a.F = e->s64.qA;
     int rc = launch_stage<double>(e, 0, 0, a);
     if (!rc) {
-        SERBF_HIP(hipMemcpy2DAsync(F_host, Nv * sizeof(double), e->s64.qA, S * sizeof(double), Nv * sizeof(double),
-                                   (size_t)N, hipMemcpyDeviceToHost, e->stream));
-        SERBF_HIP(hipStreamSynchronize(e->stream));
+        rc = download_padded(e, F_host, e->s64.qA, (size_t)N, Nv, S);
     }
     (void)hipFree(d_ts);
     (void)hipFree(d_u);

@@ -54,6 +54,18 @@ def test_matrix_core_stage_matches_column_kernel(deform60, kver):
     assert np.max(np.abs(q4[:, 3] - 1.0)) < 1e-5
 
 
+def test_matrix_core_stage_fp64(deform60):
+    """Version 5 in FP64 (v_mfma_f64_16x16x4f64) against the FP64 column kernel: the same sums
+    in another order, so they agree to a few units of FP64 rounding per step."""
+    eng, q0, dt = deform60["eng"], deform60["q0"], deform60["dt"]
+    q2 = _run(eng, q0, F64, 2, 20, dt)
+    q5 = _run(eng, q0, F64, 5, 20, dt)
+    d = np.linalg.norm(q5 - q2) / np.linalg.norm(q2)
+    print(f"kver 5 vs kver 2 (FP64): {d:.2e}")
+    assert d < 1e-12
+    assert np.max(np.abs(q5[:, 3] - 1.0)) < 1e-12
+
+
 @pytest.mark.parametrize("kver", [4, 5])
 def test_matrix_core_stage_without_hyperviscosity(deform60, kver):
     eng, cfg, q0, dt = deform60["eng"], deform60["cfg"], deform60["q0"], deform60["dt"]
@@ -65,12 +77,14 @@ def test_matrix_core_stage_without_hyperviscosity(deform60, kver):
         q64 = _run(eng, q0, F64, 2, 10, dt)
         q2 = _run(eng, q0, F32, 2, 10, dt)
         q4 = _run(eng, q0, F32, kver, 10, dt)
+        q64k = _run(eng, q0, F64, kver, 10, dt)
     finally:
         cfg.hv_coef = coef
         eng.configure(cfg)
         eng.weights()
     assert np.linalg.norm(q4 - q2) / np.linalg.norm(q2) < 1e-5
     assert np.linalg.norm(q4 - q64) / np.linalg.norm(q64) < 1e-4
+    assert np.linalg.norm(q64k - q64) / np.linalg.norm(q64) < 1e-12
 
 
 @pytest.mark.parametrize("kver", [4, 5])
