@@ -14,6 +14,7 @@ Timed legs on the same inputs:
   f32+closure  FP32 state and SpMV, hyperviscosity in every stage with the learned
                closure's per-point coefficients (re-evaluated every 32 steps)  -> value
   f32          the same with the constant coefficient (its stage kernel gives the roofline)
+Both precisions use the matrix-core tile stage kernel (kernel version 5).
 --period runs both the f64 and the f32+closure legs over the full 12-day period of the
 reversing flow and states their error against the exact solution q(T) = q(0).
 
@@ -32,6 +33,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_F32_PEAK_TFLOPS = 157.3   # dense FP32 matrix peak of MI355X
+TILE = 16                       # nodes per matrix-core tile
 
 
 def log(msg):
@@ -47,6 +50,23 @@ def stage_bytes(N, n, Nv, ntr, bq, bw, basis=16):
     Stencil: n indices (4 B) + 3 weight sets (Dx, Dy, L) per node; velocity basis
     (16 values) per node."""
     return N * (ntr * Nv * 4.25 * bq + n * (3 * bw + 4) + basis * bw)
+
+
+def tile_flops(idx, S, ntr, hv=True):
+    """Matrix-core work of one stage launch of the tile kernel: per 16-node tile the union of
+    the stencils, rounded up to 16 columns, times 16 nodes x S levels x ntr tracers x 2 flops,
+    once for the advection operator and once for Delta^k; and the useful part of it (n stencil
+    entries per node)."""
+    N, n = idx.shape
+    nt = (N + TILE - 1) // TILE
+    pad = np.full((nt * TILE - N, n), -1, idx.dtype)
+    keys = np.sort(np.concatenate([idx, pad]).reshape(nt, TILE * n), axis=1)
+    u = 1 + np.count_nonzero(np.diff(keys, axis=1), axis=1) - (keys[:, 0] < 0)
+    u16 = (u + 15) // 16 * 16
+    ops = 2 if hv else 1
+    dense = float(np.sum(u16)) * TILE * S * ntr * 2 * ops
+    useful = float(N) * n * S * ntr * 2 * ops
+    return dense, useful, float(np.mean(u))
 
 
 def load_traffic(kernel_key):
@@ -187,6 +207,9 @@ def main():
 
     kern = "fp32 step (Delta^4 in every stage)"
     traffic = load_traffic(kern)
+    idx, _ = eng.stencils()
+    dense_fl, useful_fl, umean = tile_flops(idx, 64, ntr)
+    mfma_tf = dense_fl / (res["f32"][1] / 1e3) / 1e12
     out = {
         "metric": "RK4 point-updates/s @655362x60, FP32+closure vs FP64 plain; l2 err",
         "value": value["f32+closure"],
@@ -205,7 +228,8 @@ def main():
                                "hyperviscosity",
                    "nodes": N, "levels": cfg.Nv, "tracers": ntr, "stencil": cfg.n, "dt_s": dt,
                    "point_update": "one (node, level) advanced one RK4 step (all 4 tracers)",
-                   "closure": closure_kind, "closure_every_steps": args.closure_every, "closure_s_max": s_max},
+                   "closure": closure_kind, "closure_every_steps": args.closure_every, "closure_s_max": s_max,
+                   "stage_kernel": "matrix-core tiles (kernel version 5) in both precisions"},
         "fp64_plain": {"value": value["f64"], "ms_per_step": res["f64"][0] / args.steps,
                        "stage_us": res["f64"][1] * 1e3, "roofline_achieved_GBs": achieved64,
                        "roofline_frac": achieved64 / HBM_PEAK_GBS},
@@ -217,10 +241,14 @@ def main():
         "setup": {"knn_s": knn_s, "weights_s": w_s, "weight_solves_per_s": 2 * N / w_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stage2_kernel<float, 64, 4, STAGE, HV, 8>, average of the 4 launches of a step",
+                     "kernel": "mtile2_kernel<float, STAGE, 1> (matrix-core tiles), average of the 4 launches "
+                               "of a step",
                      "bytes_per_launch": b32,
                      "note": "average FP32 stage launch of the f32 leg (HIP events); algorithmic bytes model "
-                             "bench.stage_bytes; traffic = calibrated FETCH_SIZE + WRITE_SIZE per launch"},
+                             "bench.stage_bytes; traffic = calibrated FETCH_SIZE + WRITE_SIZE per launch",
+                     "mfma_view": {"dense_flops_per_launch": dense_fl, "useful_flops_per_launch": useful_fl,
+                                   "mean_tile_union": umean, "achieved_TFLOPs": mfma_tf,
+                                   "peak_TFLOPs": MFMA_F32_PEAK_TFLOPS, "frac": mfma_tf / MFMA_F32_PEAK_TFLOPS}},
         "cpu_baseline": cpu,
         "config_c_weights": weights_c,
     }

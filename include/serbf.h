@@ -177,8 +177,9 @@ int serbf_set_gamma(serbf_engine *e, const double *gamma);
  * matrix operand (FP32 and FP64, 4 tracers of 64 levels, config-mode solid-body or
  * deformational flow, hyperviscosity in every stage; other cases use 2), 2 column-vector
  * form, 3 LDS-tiled (columns of 64 levels, 1 or 4 tracers; other shapes use 2), 4
- * matrix-core tiles with the state staged in LDS (FP32, the cases of 5), 1 lane-per-level
- * form. */
+ * matrix-core tiles with the state staged in LDS (FP32, the cases of 5), 6 version 5 as a
+ * persistent pipeline that overlaps each tile's RK4 update with the next tile's matrix-core
+ * work (the cases of 5), 1 lane-per-level form. */
 int serbf_set_kernel_version(serbf_engine *e, int version);
 /* Neighbour columns each wavefront loads before consuming them (4, 8 or 16): the
  * memory-level parallelism / register trade-off of the stage kernel. */

@@ -534,7 +534,7 @@ int serbf_set_hv_split(serbf_engine *e, int split)
 
 int serbf_set_kernel_version(serbf_engine *e, int version)
 {
-    SERBF_CHECK(e && version >= 1 && version <= 5, SERBF_ERR_ARG, "set_kernel_version: 1 to 5");
+    SERBF_CHECK(e && version >= 1 && version <= 6, SERBF_ERR_ARG, "set_kernel_version: 1 to 6");
     e->kver = version;
     return SERBF_OK;
 }

@@ -42,6 +42,14 @@ __device__ __forceinline__ v4d mfma4(double a, double b, v4d c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+// orders the LDS accesses of one wavefront
+__device__ __forceinline__ void wave_sync_m()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 template <typename T>
 struct Acc4;
 template <>
@@ -359,6 +367,238 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan t
     }
 }
 
+// ---------------------------------------------------------------------------
+// Version 6: version 5 as a persistent, wavefront-specialised pipeline.
+//
+// A work-group of 8 wavefronts walks a strided sequence of tiles of its XCD's contiguous
+// range.  Wavefronts 0-3 (the matrix group) build the dense weight blocks of tile i and run the
+// matrix-core loop of version 5 on it; wavefronts 4-7 (the update group) run the vertical stencil,
+// velocity and RK4 update of tile i-1 meanwhile.  The matrix group hands over, per point, the
+// horizontal part of the right-hand side, alpha r_inv (W q) + c (W_L q), through a double-buffered
+// LDS array.  Each tile takes two phases, each closed by a work-group barrier:
+//   phase 1: matrix group: clear and fill the weight blocks and union offsets of tile i (each
+//            matrix wavefront owns 4 rows of the blocks, so its clearing and filling are
+//            ordered within the wavefront) | update group: nodes 0-1 of each wavefront of tile i-1
+//   phase 2: matrix group: matrix-core loop of tile i, hand-over written to buffer i % 2 |
+//            update group: nodes 2-3 of each wavefront of tile i-1 (read from buffer (i-1) % 2)
+// So the HBM-bound update streams of one tile overlap the L2 gathers and matrix-core work of
+// the next, which a one-tile-per-work-group kernel leaves to chance.  The update group holds a
+// column per lane (lane = level) and takes the vertical stencil's neighbours by cross-lane
+// shuffles, so nothing of the state is staged in LDS.
+__host__ __device__ inline int mtile3_lds_bytes(int up, bool hv, int tsize)
+{
+    const int wld = up + 1;
+    const int wreg = 16 * wld * (hv ? 2 : 1);
+    const int dreg = 2 * kMTileNodes * 4 * 64;
+    return (wreg + dreg) * tsize + up * (int)sizeof(int);
+}
+
+template <typename T, int STAGE, int HV>
+__device__ __forceinline__ void m3_update_node(const StageArgs<T> &a, const T *Dbuf, const T *ts, int p0, int nd,
+                                               int v)
+{
+    constexpr int S = 64, NTR = 4;
+    const int p = p0 + nd;
+    if (p >= a.N) return;   // wavefront-uniform
+    const bool live = v < a.Nv;
+    const int vst = a.vstart[live ? v : 0];
+    T upx, upy, upz;
+    level_velocity<T, S>(a, a.basis + (size_t)p * kBasis, ts, p, live ? v : 0, upx, upy, upz);
+    (void)upx;
+    (void)upy;
+#pragma unroll
+    for (int t = 0; t < NTR; t++) {
+        const size_t my = ((size_t)p * NTR + t) * S + v;
+        const T self = a.qin[my];
+        T z = T(0);
+#pragma unroll
+        for (int jj = 0; jj < kMaxNv; jj++) {
+            const T qj = __shfl(self, (vst + jj) & 63, 64);
+            if (live && jj < a.nv) z = fma(a.vW[v * kMaxNv + jj], qj, z);
+        }
+        T f = Dbuf[(nd * NTR + t) * S + v] + a.alpha * upz * z;
+        if (!live) f = T(0);
+        if (STAGE == 0) {
+            a.F[my] = f;
+        } else if (STAGE == 1) {
+            a.F[my] = f;
+            a.qout[my] = live ? self + a.hdt * f : T(0);
+        } else if (STAGE == 2 || STAGE == 3) {
+            const T h = STAGE == 2 ? a.hdt : a.dt;
+            a.F[my] = a.F[my] + T(2) * f;
+            a.qout[my] = live ? a.qT[my] + h * f : T(0);
+        } else {
+            a.qout[my] = live ? a.qT[my] + a.dt6 * (a.F[my] + T(1) * f) : T(0);
+        }
+    }
+}
+
+template <typename T, int STAGE, int HV>
+__global__ void __launch_bounds__(512) mtile3_kernel(StageArgs<T> a, MTilePlan tp, int up, int ntiles)
+{
+    extern __shared__ __align__(16) unsigned char m3_lds[];
+    typedef typename Acc4<T>::type VT;
+    constexpr int S = 64, NTR = 4;
+    const int wld = up + 1;
+    const int wreg = 16 * wld * (HV ? 2 : 1);
+    T *W = (T *)m3_lds;
+    T *WLt = W + 16 * wld;
+    T *D = W + wreg;                                   // [2][16 nodes][4 tracers][64 levels]
+    int *uoff = (int *)(D + 2 * kMTileNodes * NTR * S);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const bool mgroup = wv < 4;
+    const int n = a.n;
+    const T *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
+
+    // this work-group's tiles: XCD x = blockIdx % 8 owns a contiguous range; its G work-groups
+    // take every G-th tile of it
+    const int x = blockIdx.x & 7, j = blockIdx.x >> 3, G = gridDim.x >> 3;
+    const int q = ntiles >> 3, r = ntiles & 7;
+    const int t_begin = x * q + (x < r ? x : r), t_end = t_begin + q + (x < r ? 1 : 0);
+    const int nmine = t_begin + j < t_end ? (t_end - 1 - (t_begin + j)) / G + 1 : 0;
+
+    for (int it = 0; it <= nmine; it++) {
+        const int tc = t_begin + j + it * G;           // matrix tile (it < nmine)
+        const int tu = tc - G;                          // update tile (it > 0)
+        // ---- phase 1 ----
+        if (mgroup) {
+            if (it < nmine) {
+                const int p0 = tc * kMTileNodes;
+                const int U = tp.ucount[tc];
+                const int u16 = (U + 15) & ~15;
+                const int *uni = tp.uni + (size_t)tc * tp.ustride;
+                for (int k = tid; k < u16; k += 256) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
+                // rows 4 wv .. 4 wv + 3: clear, then scatter this tile's stencils
+                for (int e = lane; e < 4 * wld; e += 64) {
+                    W[4 * wv * wld + e] = T(0);
+                    if (HV) WLt[4 * wv * wld + e] = T(0);
+                }
+                wave_sync_m();
+                for (int e = lane; e < 4 * n; e += 64) {
+                    const int t = 4 * wv + e / n, s = e % n;
+                    const int p = p0 + t;
+                    if (p < a.N) {
+                        const size_t o = (size_t)p * n + s;
+                        T ux, uy;
+                        node_velocity<T>(a, a.basis + (size_t)p * kBasis, ts, ux, uy);
+                        const int pos = t * wld + tp.lidx[o];
+                        W[pos] = ux * a.Wx[o] + uy * a.Wy[o];
+                        if (HV) WLt[pos] = a.WL[o];
+                    }
+                }
+            }
+        } else if (it > 0) {
+            const T *Db = D + (size_t)((it - 1) & 1) * kMTileNodes * NTR * S;
+            for (int k = 0; k < 2; k++) m3_update_node<T, STAGE, HV>(a, Db, ts, tu * kMTileNodes, 4 * (wv - 4) + k, lane);
+        }
+        __syncthreads();
+        // ---- phase 2 ----
+        if (mgroup) {
+            if (it < nmine) {
+                const int c = lane & 15, g = lane >> 4, w = wv;
+                const int p0 = tc * kMTileNodes;
+                const int u16 = (tp.ucount[tc] + 15) & ~15;
+                const int vl = 16 * w + c;
+                VT acc[NTR], accl[NTR];
+#pragma unroll
+                for (int t = 0; t < NTR; t++) {
+                    acc[t] = VT{T(0), T(0), T(0), T(0)};
+                    accl[t] = VT{T(0), T(0), T(0), T(0)};
+                }
+                const T *qw = a.qin + vl;
+                const T *wa = W + c * wld + g;
+                const T *wl = WLt + c * wld + g;
+                for (int k0 = 0; k0 < u16; k0 += 16) {
+                    int off[4];
+                    T b[4][NTR], av[4], al[4];
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++) off[jj] = uoff[k0 + 4 * jj + g];
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++)
+#pragma unroll
+                        for (int t = 0; t < NTR; t++) b[jj][t] = qw[off[jj] + t * S];
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++) {
+                        av[jj] = wa[k0 + 4 * jj];
+                        if (HV) al[jj] = wl[k0 + 4 * jj];
+                    }
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++)
+#pragma unroll
+                        for (int t = 0; t < NTR; t++) {
+                            acc[t] = mfma4(av[jj], b[jj][t], acc[t]);
+                            if (HV) accl[t] = mfma4(al[jj], b[jj][t], accl[t]);
+                        }
+                }
+                T *Db = D + (size_t)(it & 1) * kMTileNodes * NTR * S;
+                const T ar = a.alpha * a.lev[vl];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int nd = Acc4<T>::row(g, i);
+                    const int p = p0 + nd < a.N ? p0 + nd : a.N - 1;
+#pragma unroll
+                    for (int t = 0; t < NTR; t++) {
+                        T h = ar * acc[t][i];
+                        if (HV) h += (a.hv_mode ? a.gamma[((size_t)p * NTR + t) * S + vl] : a.hv_coef) * accl[t][i];
+                        Db[(nd * NTR + t) * S + vl] = h;
+                    }
+                }
+            }
+        } else if (it > 0) {
+            const T *Db = D + (size_t)((it - 1) & 1) * kMTileNodes * NTR * S;
+            for (int k = 2; k < 4; k++) m3_update_node<T, STAGE, HV>(a, Db, ts, tu * kMTileNodes, 4 * (wv - 4) + k, lane);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int STAGE, int HV>
+void launch_three(serbf_engine *e, const StageArgs<T> &a, const MTilePlan &tp, int up, int bytes, int grid)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)mtile3_kernel<T, STAGE, HV>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((mtile3_kernel<T, STAGE, HV>), dim3(grid), dim3(512), bytes, e->stream, a, tp, up, e->ntiles);
+}
+
+template <typename T>
+int launch_mtile3(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTilePlan &tp)
+{
+    const int up = (e->tile_umax + 15) & ~15;
+    const int bytes = mtile3_lds_bytes(up, hv != 0, (int)sizeof(T));
+    SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "matrix-core stage kernel: tile union too large");
+    SERBF_CHECK(e->n <= 128, SERBF_ERR_UNSUPPORTED, "matrix-core stage kernel: n > 128");
+    if (e->cus <= 0) {
+        hipDeviceProp_t prop;
+        e->cus = hipGetDeviceProperties(&prop, e->device) == hipSuccess ? prop.multiProcessorCount : 256;
+    }
+    // resident work-groups only (the kernel is persistent), a multiple of 8
+    int per_cu = 0;
+    SERBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)mtile3_kernel<T, 1, 1>, 512,
+                                                           bytes));
+    if (per_cu < 1) per_cu = 1;
+    int grid = e->cus * per_cu;
+    grid = grid / 8 * 8;
+    if (grid < 8) grid = 8;
+    switch (stage * 2 + (hv ? 1 : 0)) {
+    case 0: launch_three<T, 0, 0>(e, a, tp, up, bytes, grid); break;
+    case 1: launch_three<T, 0, 1>(e, a, tp, up, bytes, grid); break;
+    case 2: launch_three<T, 1, 0>(e, a, tp, up, bytes, grid); break;
+    case 3: launch_three<T, 1, 1>(e, a, tp, up, bytes, grid); break;
+    case 4: launch_three<T, 2, 0>(e, a, tp, up, bytes, grid); break;
+    case 5: launch_three<T, 2, 1>(e, a, tp, up, bytes, grid); break;
+    case 6: launch_three<T, 3, 0>(e, a, tp, up, bytes, grid); break;
+    case 7: launch_three<T, 3, 1>(e, a, tp, up, bytes, grid); break;
+    case 8: launch_three<T, 4, 0>(e, a, tp, up, bytes, grid); break;
+    default: launch_three<T, 4, 1>(e, a, tp, up, bytes, grid); break;
+    }
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
 template <typename T, int STAGE, int HV>
 void launch_two(serbf_engine *e, const StageArgs<T> &a, const MTilePlan &tp, int up, int bytes)
 {
@@ -416,6 +656,7 @@ int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<T> a)
     tp.lidx = e->d_lidx;
     tp.ustride = e->tile_ustride;
     a.nblk = e->ntiles;
+    if (e->kver == 6) return launch_mtile3<T>(e, stage, hv, a, tp);
     if constexpr (std::is_same<T, double>::value) {
         return launch_mtile2<double>(e, stage, hv, a, tp);
     } else {

@@ -705,7 +705,7 @@ int launch_stage(serbf_engine *e, int stage, int hv, const StageArgs<T> &a)
     if constexpr (std::is_same<T, float>::value) {
         if (e->kver == 4 && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
     }
-    if (e->kver == 5 && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
+    if ((e->kver == 5 || e->kver == 6) && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
     if (e->kver == 3 && e->have_plan && !e->plan_failed && tile_supported(e)) {
         const bool xy = a.flow == VEL_TC2 || a.flow == VEL_ARRAY || (a.ref_mode && a.flow == VEL_DEFORM);
         launch_tile<T>(e, stage, hv, xy, a);
@@ -789,7 +789,7 @@ int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, doubl
     int rc = ensure_tscal<T>(e, sb, nsteps, dt, t0);
     if (rc) return rc;
     if (e->kver == 3 && tile_supported(e) && (rc = tile_plan(e))) return rc;
-    if ((e->kver == 4 || e->kver == 5) && e->ntr == 4 && e->S == 64 && e->n <= 128 && (rc = tile_plan(e))) return rc;
+    if (e->kver >= 4 && e->ntr == 4 && e->S == 64 && e->n <= 128 && (rc = tile_plan(e))) return rc;
     const int zero = 0;
     SERBF_HIP(hipMemcpyAsync(e->d_base, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
     // the graph of kGraphSteps steps reads the step counter and bumps it at the end

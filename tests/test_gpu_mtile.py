@@ -1,6 +1,7 @@
 """The matrix-core stage kernels (16-node tiles, dense W x Q products on
-v_mfma_f32_16x16x4f32; version 4 stages the state in LDS, version 5 reads it directly as the
-matrix operand) against the column-vector kernel and the FP64 path, with and without
+v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64; version 4 stages the state in LDS, version 5
+reads it directly as the matrix operand, version 6 is version 5 as a persistent pipeline of
+matrix and update wavefronts) against the column-vector kernel and the FP64 path, with and without
 hyperviscosity and with the learned closure's per-point coefficients.  10242 nodes leave a
 partial last tile."""
 import numpy as np
@@ -38,7 +39,7 @@ def _run(eng, q0, prec, kver, steps, dt):
     return eng.state(prec)
 
 
-@pytest.mark.parametrize("kver", [4, 5])
+@pytest.mark.parametrize("kver", [4, 5, 6])
 def test_matrix_core_stage_matches_column_kernel(deform60, kver):
     eng, q0, dt = deform60["eng"], deform60["q0"], deform60["dt"]
     q64 = _run(eng, q0, F64, 2, 20, dt)
@@ -54,19 +55,20 @@ def test_matrix_core_stage_matches_column_kernel(deform60, kver):
     assert np.max(np.abs(q4[:, 3] - 1.0)) < 1e-5
 
 
-def test_matrix_core_stage_fp64(deform60):
-    """Version 5 in FP64 (v_mfma_f64_16x16x4f64) against the FP64 column kernel: the same sums
-    in another order, so they agree to a few units of FP64 rounding per step."""
+@pytest.mark.parametrize("kver", [5, 6])
+def test_matrix_core_stage_fp64(deform60, kver):
+    """Versions 5 and 6 in FP64 (v_mfma_f64_16x16x4f64) against the FP64 column kernel: the
+    same sums in another order, so they agree to a few units of FP64 rounding per step."""
     eng, q0, dt = deform60["eng"], deform60["q0"], deform60["dt"]
     q2 = _run(eng, q0, F64, 2, 20, dt)
-    q5 = _run(eng, q0, F64, 5, 20, dt)
+    q5 = _run(eng, q0, F64, kver, 20, dt)
     d = np.linalg.norm(q5 - q2) / np.linalg.norm(q2)
-    print(f"kver 5 vs kver 2 (FP64): {d:.2e}")
+    print(f"kver {kver} vs kver 2 (FP64): {d:.2e}")
     assert d < 1e-12
     assert np.max(np.abs(q5[:, 3] - 1.0)) < 1e-12
 
 
-@pytest.mark.parametrize("kver", [4, 5])
+@pytest.mark.parametrize("kver", [4, 5, 6])
 def test_matrix_core_stage_without_hyperviscosity(deform60, kver):
     eng, cfg, q0, dt = deform60["eng"], deform60["cfg"], deform60["q0"], deform60["dt"]
     coef = cfg.hv_coef
@@ -87,7 +89,7 @@ def test_matrix_core_stage_without_hyperviscosity(deform60, kver):
     assert np.linalg.norm(q64k - q64) / np.linalg.norm(q64) < 1e-12
 
 
-@pytest.mark.parametrize("kver", [4, 5])
+@pytest.mark.parametrize("kver", [4, 5, 6])
 def test_matrix_core_stage_with_closure(deform60, kver):
     eng, q0, dt = deform60["eng"], deform60["q0"], deform60["dt"]
     rng = np.random.default_rng(4)
