@@ -55,14 +55,16 @@ __device__ __forceinline__ unsigned short f2bf(float f)
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (unsigned short)(u >> 16);
 }
+typedef float fl2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+struct Bf2x2 {
+    bf2 lo, hi;
+};
+// four floats to BF16, round to nearest even (v_cvt_pk_bf16_f32, two per instruction)
 __device__ __forceinline__ v4s pack4(float a, float b, float c, float d)
 {
-    v4s r;
-    r[0] = (short)f2bf(a);
-    r[1] = (short)f2bf(b);
-    r[2] = (short)f2bf(c);
-    r[3] = (short)f2bf(d);
-    return r;
+    const Bf2x2 t = {__builtin_convertvector((fl2){a, b}, bf2), __builtin_convertvector((fl2){c, d}, bf2)};
+    return __builtin_bit_cast(v4s, t);
 }
 __device__ __forceinline__ v4s ld4(const unsigned short *p) { return *(const v4s *)p; }
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
@@ -71,12 +73,13 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
 }
 __device__ __forceinline__ float softplus_f(float o) { return o > 20.f ? o : log1pf(expf(o)); }
 __device__ __forceinline__ float sigmoid_f(float o) { return 1.f / (1.f + expf(-o)); }
-// tanh(x) = sign(x) (1 - 2 / (exp(2|x|) + 1)) with the hardware exp and reciprocal: absolute
-// error ~1e-7, far below the BF16 rounding of the next layer's operand
+// tanh(x) = 1 - 2 / (2^(2x / ln 2) + 1) with the hardware exp2 and reciprocal (v_exp_f32,
+// v_rcp_f32): absolute error ~2e-7, far below the BF16 rounding of the next layer's operand;
+// exp2 overflowing to +inf gives 1, underflowing to 0 gives -1
 __device__ __forceinline__ float tanh_fast(float x)
 {
-    const float e = __expf(2.f * fabsf(x));
-    return copysignf(1.f - 2.f * __frcp_rn(e + 1.f), x);
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+    return fmaf(-2.f, __builtin_amdgcn_rcpf(e + 1.f), 1.f);
 }
 
 __device__ __forceinline__ void wave_sync()
@@ -128,14 +131,16 @@ __device__ __forceinline__ void load_fwd(const NetView &nv, int lane, FwdRegs &r
 // [sample c], h2[t][i] = H2[sample 4g + i][unit 16t + c], o[i] = output of sample 4g + i.
 __device__ __forceinline__ void fwd_tile(const FwdRegs &r, v4s xb, float h1[4][4], float h2[4][4], float o[4])
 {
-    const v4f zero = {0.f, 0.f, 0.f, 0.f};
+    // the biases are the accumulators' initial values
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const v4f z = mfma(r.w1[t], xb, zero);
+        const v4f z = mfma(r.w1[t], xb, v4f{r.b1[t][0], r.b1[t][1], r.b1[t][2], r.b1[t][3]});
 #pragma unroll
-        for (int i = 0; i < 4; i++) h1[t][i] = tanh_fast(z[i] + r.b1[t][i]);
+        for (int i = 0; i < 4; i++) h1[t][i] = tanh_fast(z[i]);
     }
-    v4f z2[4] = {zero, zero, zero, zero};
+    v4f z2[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) z2[t] = v4f{r.b2[t], r.b2[t], r.b2[t], r.b2[t]};
 #pragma unroll
     for (int kk = 0; kk < 4; kk++) {
         const v4s a = pack4(h1[kk][0], h1[kk][1], h1[kk][2], h1[kk][3]);
@@ -145,7 +150,7 @@ __device__ __forceinline__ void fwd_tile(const FwdRegs &r, v4s xb, float h1[4][4
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) h2[t][i] = tanh_fast(z2[t][i] + r.b2[t]);
+        for (int i = 0; i < 4; i++) h2[t][i] = tanh_fast(z2[t][i]);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         float s = 0.f;
@@ -395,12 +400,12 @@ __global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T
             f[7] = fc[3];
             f[8] = (float)own[vv];
             f[9] = (float)asinh(K * (T)a.dh * fabs(dz));
-            unsigned short *xrow = xs + (t * 64 + lane) * 16;
+            float xn[16];
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const float x = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
-                xrow[k] = f2bf(x);
-            }
+            for (int k = 0; k < 16; k++) xn[k] = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
+            v4s *xrow = (v4s *)(xs + (t * 64 + lane) * 16);
+#pragma unroll
+            for (int k = 0; k < 4; k++) xrow[k] = pack4(xn[4 * k], xn[4 * k + 1], xn[4 * k + 2], xn[4 * k + 3]);
         }
         wave_sync();
         // the network operands are (re)loaded after the features, so the two phases do not

@@ -222,12 +222,14 @@ __host__ __device__ inline int mtile2_lds_bytes(int up, bool hv)
     return (32 + (wreg > oreg ? wreg : oreg)) * (int)sizeof(T) + up * (int)sizeof(int);
 }
 
-template <typename T, int STAGE, int HV>
-__global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan tp, int up)
+// TPW tracers per wavefront: 4 (4 wavefronts per work-group) or 2 (8 wavefronts; wavefront w
+// takes level block w % 4 of tracers 2 (w / 4), 2 (w / 4) + 1)
+template <typename T, int STAGE, int HV, int TPW>
+__global__ void __launch_bounds__(256 * (4 / TPW)) mtile2_kernel(StageArgs<T> a, MTilePlan tp, int up)
 {
     extern __shared__ __align__(16) unsigned char m2_lds[];
     typedef typename Acc4<T>::type VT;
-    constexpr int S = 64, NTR = 4;
+    constexpr int S = 64, NTR = 4, NTH = 256 * (4 / TPW);
     const int wld = up + 1;
     const int wreg = 16 * wld * (HV ? 2 : 1), oreg = kMTileNodes * NTR * kOwnLD;
     T *sv = (T *)m2_lds;
@@ -235,7 +237,7 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan t
     T *WLt = W + 16 * wld;
     T *own = W;                                       // reused after the main loop
     int *uoff = (int *)(W + (wreg > oreg ? wreg : oreg));
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = (tid >> 6) & 3, t0 = (tid >> 8) * TPW;
     const int c = lane & 15, g = lane >> 4;
     const int tile = xcd_remap(blockIdx.x, a.nblk);
     const int p0 = tile * kMTileNodes;
@@ -253,10 +255,10 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan t
         sv[2 * tid] = ux;
         sv[2 * tid + 1] = uy;
     }
-    for (int k = tid; k < u16; k += 256) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
-    for (int e = tid; e < wreg; e += 256) W[e] = T(0);
+    for (int k = tid; k < u16; k += NTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
+    for (int e = tid; e < wreg; e += NTH) W[e] = T(0);
     __syncthreads();
-    for (int e = tid; e < kMTileNodes * n; e += 256) {
+    for (int e = tid; e < kMTileNodes * n; e += NTH) {
         const int t = e / n, s = e - t * n;
         const int p = p0 + t;
         if (p < a.N) {
@@ -269,24 +271,24 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan t
     __syncthreads();
 
     // ---- D (16 nodes x 16 levels of wavefront w) = W (16 x u16) x Q (u16 x 16), per tracer ----
-    VT acc[NTR], accl[NTR];
+    VT acc[TPW], accl[TPW];
 #pragma unroll
-    for (int t = 0; t < NTR; t++) {
+    for (int t = 0; t < TPW; t++) {
         acc[t] = VT{T(0), T(0), T(0), T(0)};
         accl[t] = VT{T(0), T(0), T(0), T(0)};
     }
-    const T *qw = a.qin + 16 * w + c;
+    const T *qw = a.qin + t0 * S + 16 * w + c;
     const T *wa = W + c * wld + g;
     const T *wl = WLt + c * wld + g;
     for (int k0 = 0; k0 < u16; k0 += 16) {
         int off[4];
-        T b[4][NTR], av[4], al[4];
+        T b[4][TPW], av[4], al[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) off[j] = uoff[k0 + 4 * j + g];
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
-            for (int t = 0; t < NTR; t++) b[j][t] = qw[off[j] + t * S];
+            for (int t = 0; t < TPW; t++) b[j][t] = qw[off[j] + t * S];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             av[j] = wa[k0 + 4 * j];
@@ -295,7 +297,7 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan t
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
-            for (int t = 0; t < NTR; t++) {
+            for (int t = 0; t < TPW; t++) {
                 acc[t] = mfma4(av[j], b[j][t], acc[t]);
                 if (HV) accl[t] = mfma4(al[j], b[j][t], accl[t]);
             }
@@ -303,7 +305,7 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan t
 
     // ---- own columns of the tile in LDS, 8 zero levels on either side ----
     __syncthreads();   // every wavefront is done with W and uoff
-    for (int e = tid; e < kMTileNodes * NTR * (S / 4); e += 256) {
+    for (int e = tid; e < kMTileNodes * NTR * (S / 4); e += NTH) {
         const int col = e >> 4, l4 = (e & 15) * 4;
         const int t = col / NTR, tr = col - t * NTR;
         const int p = p0 + t < a.N ? p0 + t : a.N - 1;
@@ -311,7 +313,7 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan t
         T *d = own + col * kOwnLD + 8 + l4;
         d[0] = x.v[0]; d[1] = x.v[1]; d[2] = x.v[2]; d[3] = x.v[3];
     }
-    for (int e = tid; e < kMTileNodes * NTR * 16; e += 256) {
+    for (int e = tid; e < kMTileNodes * NTR * 16; e += NTH) {
         const int col = e >> 4, k = e & 15;
         own[col * kOwnLD + (k < 8 ? k : S + k)] = T(0);
     }
@@ -322,46 +324,62 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan t
     const bool live = v < a.Nv;
     const T r_inv = a.lev[v];
     const int vst = a.vstart[live ? v : 0];
-    T z[4][NTR];
+    T z[4][TPW];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int t = 0; t < NTR; t++) z[i][t] = T(0);
+        for (int t = 0; t < TPW; t++) z[i][t] = T(0);
     if (live)
         for (int j = 0; j < a.nv; j++) {
             const T wv = a.vW[v * kMaxNv + j];
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int t = 0; t < NTR; t++)
-                    z[i][t] = fma(wv, own[(Acc4<T>::row(g, i) * NTR + t) * kOwnLD + 8 + vst + j], z[i][t]);
+                for (int t = 0; t < TPW; t++)
+                    z[i][t] = fma(wv, own[(Acc4<T>::row(g, i) * NTR + t0 + t) * kOwnLD + 8 + vst + j], z[i][t]);
         }
+    // every load of the update is issued before its first store: the stores may alias the
+    // loaded arrays as far as the compiler knows, so loads placed after a store would wait for
+    // one memory round trip per output
+    T Fv[4][TPW], Qv[4][TPW], Gv[4][TPW], upz[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int pr = p0 + Acc4<T>::row(g, i);
+        const int p = pr < a.N ? pr : a.N - 1;
+        T upx, upy;
+        level_velocity<T, S>(a, a.basis + (size_t)p * kBasis, ts, p, v, upx, upy, upz[i]);
+        (void)upx;
+        (void)upy;
+#pragma unroll
+        for (int t = 0; t < TPW; t++) {
+            const size_t my = ((size_t)p * NTR + t0 + t) * S + v;
+            Gv[i][t] = HV && a.hv_mode ? a.gamma[my] : a.hv_coef;
+            Fv[i][t] = STAGE >= 2 ? a.F[my] : T(0);
+            Qv[i][t] = STAGE >= 2 ? a.qT[my] : T(0);
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int nd = Acc4<T>::row(g, i);
         const int p = p0 + nd;
         if (p >= a.N) continue;
-        T upx, upy, upz;
-        level_velocity<T, S>(a, a.basis + (size_t)p * kBasis, ts, p, v, upx, upy, upz);
-        (void)upx;
-        (void)upy;
 #pragma unroll
-        for (int t = 0; t < NTR; t++) {
-            const size_t my = ((size_t)p * NTR + t) * S + v;
-            T f = a.alpha * (acc[t][i] * r_inv + upz * z[i][t]);
-            if (HV) f += (a.hv_mode ? a.gamma[my] : a.hv_coef) * accl[t][i];
+        for (int t = 0; t < TPW; t++) {
+            const size_t my = ((size_t)p * NTR + t0 + t) * S + v;
+            T f = a.alpha * (acc[t][i] * r_inv + upz[i] * z[i][t]);
+            if (HV) f += Gv[i][t] * accl[t][i];
             if (!live) f = T(0);
             if (STAGE == 0) {
                 a.F[my] = f;
             } else if (STAGE == 1) {
                 a.F[my] = f;
-                a.qout[my] = live ? own[(nd * NTR + t) * kOwnLD + 8 + v] + a.hdt * f : T(0);
+                a.qout[my] = live ? own[(nd * NTR + t0 + t) * kOwnLD + 8 + v] + a.hdt * f : T(0);
             } else if (STAGE == 2 || STAGE == 3) {
                 const T h = STAGE == 2 ? a.hdt : a.dt;
-                a.F[my] = a.F[my] + T(2) * f;
-                a.qout[my] = live ? a.qT[my] + h * f : T(0);
+                a.F[my] = Fv[i][t] + T(2) * f;
+                a.qout[my] = live ? Qv[i][t] + h * f : T(0);
             } else {
-                a.qout[my] = live ? a.qT[my] + a.dt6 * (a.F[my] + T(1) * f) : T(0);
+                a.qout[my] = live ? Qv[i][t] + a.dt6 * (Fv[i][t] + T(1) * f) : T(0);
             }
         }
     }
@@ -370,14 +388,16 @@ __global__ void __launch_bounds__(256) mtile2_kernel(StageArgs<T> a, MTilePlan t
 // ---------------------------------------------------------------------------
 // Version 6: version 5 as a persistent, wavefront-specialised pipeline.
 //
-// A work-group of 8 wavefronts walks a strided sequence of tiles of its XCD's contiguous
-// range.  Wavefronts 0-3 (the matrix group) build the dense weight blocks of tile i and run the
-// matrix-core loop of version 5 on it; wavefronts 4-7 (the update group) run the vertical stencil,
+// A work-group of 12 wavefronts walks a strided sequence of tiles of its XCD's contiguous
+// range.  Wavefronts 0-7 (the matrix group; wavefront w takes level block w % 4 of tracers
+// 2 (w / 4) and 2 (w / 4) + 1) build the dense weight blocks of tile i and run the matrix-core
+// loop of version 5 on it, with the union-column loads of the next step in flight during the
+// MFMAs of the current one; wavefronts 8-11 (the update group) run the vertical stencil,
 // velocity and RK4 update of tile i-1 meanwhile.  The matrix group hands over, per point, the
 // horizontal part of the right-hand side, alpha r_inv (W q) + c (W_L q), through a double-buffered
 // LDS array.  Each tile takes two phases, each closed by a work-group barrier:
 //   phase 1: matrix group: clear and fill the weight blocks and union offsets of tile i (each
-//            matrix wavefront owns 4 rows of the blocks, so its clearing and filling are
+//            matrix wavefront owns 2 rows of the blocks, so its clearing and filling are
 //            ordered within the wavefront) | update group: nodes 0-1 of each wavefront of tile i-1
 //   phase 2: matrix group: matrix-core loop of tile i, hand-over written to buffer i % 2 |
 //            update group: nodes 2-3 of each wavefront of tile i-1 (read from buffer (i-1) % 2)
@@ -406,10 +426,19 @@ __device__ __forceinline__ void m3_update_node(const StageArgs<T> &a, const T *D
     level_velocity<T, S>(a, a.basis + (size_t)p * kBasis, ts, p, live ? v : 0, upx, upy, upz);
     (void)upx;
     (void)upy;
+    // all loads before the first store (see the version-5 update)
+    T selfv[NTR], Fv[NTR], Qv[NTR];
 #pragma unroll
     for (int t = 0; t < NTR; t++) {
         const size_t my = ((size_t)p * NTR + t) * S + v;
-        const T self = a.qin[my];
+        selfv[t] = a.qin[my];
+        Fv[t] = STAGE >= 2 ? a.F[my] : T(0);
+        Qv[t] = STAGE >= 2 ? a.qT[my] : T(0);
+    }
+#pragma unroll
+    for (int t = 0; t < NTR; t++) {
+        const size_t my = ((size_t)p * NTR + t) * S + v;
+        const T self = selfv[t];
         T z = T(0);
 #pragma unroll
         for (int jj = 0; jj < kMaxNv; jj++) {
@@ -425,20 +454,20 @@ __device__ __forceinline__ void m3_update_node(const StageArgs<T> &a, const T *D
             a.qout[my] = live ? self + a.hdt * f : T(0);
         } else if (STAGE == 2 || STAGE == 3) {
             const T h = STAGE == 2 ? a.hdt : a.dt;
-            a.F[my] = a.F[my] + T(2) * f;
-            a.qout[my] = live ? a.qT[my] + h * f : T(0);
+            a.F[my] = Fv[t] + T(2) * f;
+            a.qout[my] = live ? Qv[t] + h * f : T(0);
         } else {
-            a.qout[my] = live ? a.qT[my] + a.dt6 * (a.F[my] + T(1) * f) : T(0);
+            a.qout[my] = live ? Qv[t] + a.dt6 * (Fv[t] + T(1) * f) : T(0);
         }
     }
 }
 
 template <typename T, int STAGE, int HV>
-__global__ void __launch_bounds__(512) mtile3_kernel(StageArgs<T> a, MTilePlan tp, int up, int ntiles)
+__global__ void __launch_bounds__(768) mtile3_kernel(StageArgs<T> a, MTilePlan tp, int up, int ntiles)
 {
     extern __shared__ __align__(16) unsigned char m3_lds[];
     typedef typename Acc4<T>::type VT;
-    constexpr int S = 64, NTR = 4;
+    constexpr int S = 64, NTR = 4, MW = 8, TPW = 2, MTH = 64 * MW;
     const int wld = up + 1;
     const int wreg = 16 * wld * (HV ? 2 : 1);
     T *W = (T *)m3_lds;
@@ -446,7 +475,7 @@ __global__ void __launch_bounds__(512) mtile3_kernel(StageArgs<T> a, MTilePlan t
     T *D = W + wreg;                                   // [2][16 nodes][4 tracers][64 levels]
     int *uoff = (int *)(D + 2 * kMTileNodes * NTR * S);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const bool mgroup = wv < 4;
+    const bool mgroup = wv < MW;
     const int n = a.n;
     const T *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
 
@@ -467,15 +496,15 @@ __global__ void __launch_bounds__(512) mtile3_kernel(StageArgs<T> a, MTilePlan t
                 const int U = tp.ucount[tc];
                 const int u16 = (U + 15) & ~15;
                 const int *uni = tp.uni + (size_t)tc * tp.ustride;
-                for (int k = tid; k < u16; k += 256) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
-                // rows 4 wv .. 4 wv + 3: clear, then scatter this tile's stencils
-                for (int e = lane; e < 4 * wld; e += 64) {
-                    W[4 * wv * wld + e] = T(0);
-                    if (HV) WLt[4 * wv * wld + e] = T(0);
+                for (int k = tid; k < u16; k += MTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
+                // rows 2 wv, 2 wv + 1: clear, then scatter this tile's stencils
+                for (int e = lane; e < 2 * wld; e += 64) {
+                    W[2 * wv * wld + e] = T(0);
+                    if (HV) WLt[2 * wv * wld + e] = T(0);
                 }
                 wave_sync_m();
-                for (int e = lane; e < 4 * n; e += 64) {
-                    const int t = 4 * wv + e / n, s = e % n;
+                for (int e = lane; e < 2 * n; e += 64) {
+                    const int t = 2 * wv + e / n, s = e % n;
                     const int p = p0 + t;
                     if (p < a.N) {
                         const size_t o = (size_t)p * n + s;
@@ -489,34 +518,44 @@ __global__ void __launch_bounds__(512) mtile3_kernel(StageArgs<T> a, MTilePlan t
             }
         } else if (it > 0) {
             const T *Db = D + (size_t)((it - 1) & 1) * kMTileNodes * NTR * S;
-            for (int k = 0; k < 2; k++) m3_update_node<T, STAGE, HV>(a, Db, ts, tu * kMTileNodes, 4 * (wv - 4) + k, lane);
+            for (int k = 0; k < 2; k++) m3_update_node<T, STAGE, HV>(a, Db, ts, tu * kMTileNodes, 4 * (wv - MW) + k, lane);
         }
         __syncthreads();
         // ---- phase 2 ----
         if (mgroup) {
             if (it < nmine) {
-                const int c = lane & 15, g = lane >> 4, w = wv;
+                const int c = lane & 15, g = lane >> 4, w = wv & 3, t0 = (wv >> 2) * TPW;
                 const int p0 = tc * kMTileNodes;
                 const int u16 = (tp.ucount[tc] + 15) & ~15;
                 const int vl = 16 * w + c;
-                VT acc[NTR], accl[NTR];
+                VT acc[TPW], accl[TPW];
 #pragma unroll
-                for (int t = 0; t < NTR; t++) {
+                for (int t = 0; t < TPW; t++) {
                     acc[t] = VT{T(0), T(0), T(0), T(0)};
                     accl[t] = VT{T(0), T(0), T(0), T(0)};
                 }
-                const T *qw = a.qin + vl;
+                const T *qw = a.qin + t0 * S + vl;
                 const T *wa = W + c * wld + g;
                 const T *wl = WLt + c * wld + g;
+                T b[4][TPW];
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    const int o = uoff[4 * jj + g];
+#pragma unroll
+                    for (int t = 0; t < TPW; t++) b[jj][t] = qw[o + t * S];
+                }
                 for (int k0 = 0; k0 < u16; k0 += 16) {
-                    int off[4];
-                    T b[4][NTR], av[4], al[4];
+                    // the next step's loads are issued before this step's MFMAs
+                    T bn[4][TPW], av[4], al[4];
+                    const bool more = k0 + 16 < u16;   // wavefront-uniform
+                    if (more) {
 #pragma unroll
-                    for (int jj = 0; jj < 4; jj++) off[jj] = uoff[k0 + 4 * jj + g];
+                        for (int jj = 0; jj < 4; jj++) {
+                            const int o = uoff[k0 + 16 + 4 * jj + g];
 #pragma unroll
-                    for (int jj = 0; jj < 4; jj++)
-#pragma unroll
-                        for (int t = 0; t < NTR; t++) b[jj][t] = qw[off[jj] + t * S];
+                            for (int t = 0; t < TPW; t++) bn[jj][t] = qw[o + t * S];
+                        }
+                    }
 #pragma unroll
                     for (int jj = 0; jj < 4; jj++) {
                         av[jj] = wa[k0 + 4 * jj];
@@ -525,10 +564,16 @@ __global__ void __launch_bounds__(512) mtile3_kernel(StageArgs<T> a, MTilePlan t
 #pragma unroll
                     for (int jj = 0; jj < 4; jj++)
 #pragma unroll
-                        for (int t = 0; t < NTR; t++) {
+                        for (int t = 0; t < TPW; t++) {
                             acc[t] = mfma4(av[jj], b[jj][t], acc[t]);
                             if (HV) accl[t] = mfma4(al[jj], b[jj][t], accl[t]);
                         }
+                    if (more) {
+#pragma unroll
+                        for (int jj = 0; jj < 4; jj++)
+#pragma unroll
+                            for (int t = 0; t < TPW; t++) b[jj][t] = bn[jj][t];
+                    }
                 }
                 T *Db = D + (size_t)(it & 1) * kMTileNodes * NTR * S;
                 const T ar = a.alpha * a.lev[vl];
@@ -537,16 +582,17 @@ __global__ void __launch_bounds__(512) mtile3_kernel(StageArgs<T> a, MTilePlan t
                     const int nd = Acc4<T>::row(g, i);
                     const int p = p0 + nd < a.N ? p0 + nd : a.N - 1;
 #pragma unroll
-                    for (int t = 0; t < NTR; t++) {
+                    for (int t = 0; t < TPW; t++) {
                         T h = ar * acc[t][i];
-                        if (HV) h += (a.hv_mode ? a.gamma[((size_t)p * NTR + t) * S + vl] : a.hv_coef) * accl[t][i];
-                        Db[(nd * NTR + t) * S + vl] = h;
+                        if (HV)
+                            h += (a.hv_mode ? a.gamma[((size_t)p * NTR + t0 + t) * S + vl] : a.hv_coef) * accl[t][i];
+                        Db[(nd * NTR + t0 + t) * S + vl] = h;
                     }
                 }
             }
         } else if (it > 0) {
             const T *Db = D + (size_t)((it - 1) & 1) * kMTileNodes * NTR * S;
-            for (int k = 2; k < 4; k++) m3_update_node<T, STAGE, HV>(a, Db, ts, tu * kMTileNodes, 4 * (wv - 4) + k, lane);
+            for (int k = 2; k < 4; k++) m3_update_node<T, STAGE, HV>(a, Db, ts, tu * kMTileNodes, 4 * (wv - MW) + k, lane);
         }
         __syncthreads();
     }
@@ -561,7 +607,7 @@ void launch_three(serbf_engine *e, const StageArgs<T> &a, const MTilePlan &tp, i
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((mtile3_kernel<T, STAGE, HV>), dim3(grid), dim3(512), bytes, e->stream, a, tp, up, e->ntiles);
+    hipLaunchKernelGGL((mtile3_kernel<T, STAGE, HV>), dim3(grid), dim3(768), bytes, e->stream, a, tp, up, e->ntiles);
 }
 
 template <typename T>
@@ -577,7 +623,7 @@ int launch_mtile3(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTil
     }
     // resident work-groups only (the kernel is persistent), a multiple of 8
     int per_cu = 0;
-    SERBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)mtile3_kernel<T, 1, 1>, 512,
+    SERBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)mtile3_kernel<T, 1, 1>, 768,
                                                            bytes));
     if (per_cu < 1) per_cu = 1;
     int grid = e->cus * per_cu;
@@ -599,16 +645,24 @@ int launch_mtile3(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTil
     return SERBF_OK;
 }
 
+// tracers per wavefront of the version-5 kernel: 2 (8 wavefronts per work-group).  Measured at
+// 655362 x 60 x 4 with the batched update loads: FP32 1.44 ms per stage (4 tracers per
+// wavefront: 1.44 ms at 4 wavefronts per SIMD), FP64 3.08 ms (4 tracers: 3.57 ms, one
+// wavefront per SIMD for lack of registers)
+template <typename T>
+constexpr int kTPW = 2;
+
 template <typename T, int STAGE, int HV>
 void launch_two(serbf_engine *e, const StageArgs<T> &a, const MTilePlan &tp, int up, int bytes)
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)mtile2_kernel<T, STAGE, HV>,
+        (void)hipFuncSetAttribute((const void *)mtile2_kernel<T, STAGE, HV, kTPW<T>>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((mtile2_kernel<T, STAGE, HV>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
+    hipLaunchKernelGGL((mtile2_kernel<T, STAGE, HV, kTPW<T>>), dim3(e->ntiles), dim3(256 * (4 / kTPW<T>)), bytes,
+                       e->stream, a, tp, up);
 }
 
 template <typename T>
