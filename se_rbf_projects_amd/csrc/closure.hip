@@ -284,8 +284,8 @@ struct ApplyGeom {
 };
 
 template <typename T, int S>
-__global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, T hv_coef, float s_max,
-                                                    int every, int npad)
+__global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef,
+                                                    float s_max, int every, int npad)
 {
     using G = ApplyGeom<T, S>;
     constexpr int NPW = G::NPW, TRC = G::TRC, UB = 8;
@@ -427,7 +427,10 @@ __global__ void __launch_bounds__(256) apply_kernel(FeatArgs<T> a, NetView nv, T
             float sv = softplus_f(os[t * 64 + lane]);
             if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
             if (sv > s_max) sv = s_max;
-            gamma[((size_t)p * NT + tr0 + t) * S + v] = v < s.Nv ? hv_coef * (T)sv : T(0);
+            if (v >= s.Nv) sv = 0.f;
+            const size_t o = ((size_t)p * NT + tr0 + t) * S + v;
+            if (s16) s16[o] = (_Float16)sv;
+            else gamma[o] = hv_coef * (T)sv;
         }
         wave_sync();   // the LDS of the pass is rewritten by the next one
     }
@@ -737,7 +740,7 @@ int ensure_hloc(serbf_engine *e)
 
 template <typename T, int S>
 void launch_apply_s(serbf_engine *e, dim3 grid, size_t bytes, const FeatArgs<T> &a, const NetView &nv, T *gamma,
-                    T hv, float smax, int every, int npad)
+                    _Float16 *s16, T hv, float smax, int every, int npad)
 {
     static bool attr = false;
     if (!attr) {
@@ -745,11 +748,13 @@ void launch_apply_s(serbf_engine *e, dim3 grid, size_t bytes, const FeatArgs<T> 
                                   160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((apply_kernel<T, S>), grid, dim3(256), bytes, e->stream, a, nv, gamma, hv, smax, every, npad);
+    hipLaunchKernelGGL((apply_kernel<T, S>), grid, dim3(256), bytes, e->stream, a, nv, gamma, s16, hv, smax, every,
+                       npad);
 }
 
 template <typename T>
-int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma, int every)
+// writes the coefficients to gamma, or (s16 != null) the multipliers in FP16 to s16
+int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma, int every, _Float16 *s16 = nullptr)
 {
     const int npad = (e->n + 7) / 8 * 8;
     const int npw = 64 / e->S;
@@ -768,11 +773,11 @@ int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma, int every)
     if (e->S == 16) {
         const size_t bytes = 4 * ApplyGeom<T, 16>::wave_bytes(npad);
         SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "closure: stencil too large for the LDS");
-        launch_apply_s<T, 16>(e, grid, bytes, a, nv, gamma, hv, smax, every, npad);
+        launch_apply_s<T, 16>(e, grid, bytes, a, nv, gamma, s16, hv, smax, every, npad);
     } else {
         const size_t bytes = 4 * ApplyGeom<T, 64>::wave_bytes(npad);
         SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "closure: stencil too large for the LDS");
-        launch_apply_s<T, 64>(e, grid, bytes, a, nv, gamma, hv, smax, every, npad);
+        launch_apply_s<T, 64>(e, grid, bytes, a, nv, gamma, s16, hv, smax, every, npad);
     }
     SERBF_HIP(hipGetLastError());
     return SERBF_OK;
@@ -789,19 +794,23 @@ int alloc_gamma(serbf_engine *e)
         SERBF_HIP(hipMalloc((void **)&e->s32.gamma, P * sizeof(float)));
         SERBF_HIP(hipMemsetAsync(e->s32.gamma, 0, P * sizeof(float), e->stream));
     }
+    if (!e->s32.s16) {
+        SERBF_HIP(hipMalloc((void **)&e->s32.s16, P * sizeof(_Float16)));
+        SERBF_HIP(hipMemsetAsync(e->s32.s16, 0, P * sizeof(_Float16), e->stream));
+    }
     return SERBF_OK;
 }
 
 }  // namespace
 
 template <typename T>
-int closure_enqueue(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off)
+int closure_enqueue(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off, bool compact)
 {
     const FeatArgs<T> a = feat_args<T>(e, sb, sb.qT, dt, sb.tscal, e->d_base, step_off);
-    return launch_apply<T>(e, a, sb.gamma, e->closure_every);
+    return launch_apply<T>(e, a, sb.gamma, e->closure_every, compact ? sb.s16 : nullptr);
 }
-template int closure_enqueue<float>(serbf_engine *, StateBuffers<float> &, double, int);
-template int closure_enqueue<double>(serbf_engine *, StateBuffers<double> &, double, int);
+template int closure_enqueue<float>(serbf_engine *, StateBuffers<float> &, double, int, bool);
+template int closure_enqueue<double>(serbf_engine *, StateBuffers<double> &, double, int, bool);
 
 }  // namespace serbf
 

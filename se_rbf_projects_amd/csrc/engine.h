@@ -57,6 +57,7 @@ struct StateBuffers {
     T *tscal = nullptr;   // [steps][4 stages][kTimeScalars]
     T *gamma = nullptr;   // per-point hyperviscosity coefficient [N][ntr][S] (closure)
     T *WL1 = nullptr, *WL2 = nullptr;  // Delta, Delta^2 weights [N][n] (closure features)
+    _Float16 *s16 = nullptr;           // closure multiplier in FP16 [N][ntr][S] (FP32 tile kernel)
 };
 
 struct GraphCache {
@@ -179,5 +180,5 @@ int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha
 void time_scalars_public(const serbf_config &c, double t, double *out);
 // closure (closure.hip): coefficients from the step-start state sb.qT into sb.gamma
 template <typename T>
-int closure_enqueue(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off);
+int closure_enqueue(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off, bool compact);
 }  // namespace serbf

@@ -342,22 +342,41 @@ __global__ void __launch_bounds__(256 * (4 / TPW)) mtile2_kernel(StageArgs<T> a,
     // loaded arrays as far as the compiler knows, so loads placed after a store would wait for
     // one memory round trip per output
     T Fv[4][TPW], Qv[4][TPW], Gv[4][TPW], upz[4];
+    size_t mys[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int pr = p0 + Acc4<T>::row(g, i);
         const int p = pr < a.N ? pr : a.N - 1;
+        mys[i] = ((size_t)p * NTR + t0) * S + v;
         T upx, upy;
         level_velocity<T, S>(a, a.basis + (size_t)p * kBasis, ts, p, v, upx, upy, upz[i]);
         (void)upx;
         (void)upy;
+    }
+    // the coefficient source is the same for the whole launch
+    if (HV && a.hv_mode == 2) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int t = 0; t < TPW; t++) Gv[i][t] = a.hv_coef * (T)(float)a.s16[mys[i] + t * S];
+    } else if (HV && a.hv_mode) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int t = 0; t < TPW; t++) Gv[i][t] = a.gamma[mys[i] + t * S];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int t = 0; t < TPW; t++) Gv[i][t] = a.hv_coef;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int t = 0; t < TPW; t++) {
-            const size_t my = ((size_t)p * NTR + t0 + t) * S + v;
-            Gv[i][t] = HV && a.hv_mode ? a.gamma[my] : a.hv_coef;
-            Fv[i][t] = STAGE >= 2 ? a.F[my] : T(0);
-            Qv[i][t] = STAGE >= 2 ? a.qT[my] : T(0);
+            Fv[i][t] = STAGE >= 2 ? a.F[mys[i] + t * S] : T(0);
+            Qv[i][t] = STAGE >= 2 ? a.qT[mys[i] + t * S] : T(0);
         }
-    }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int nd = Acc4<T>::row(g, i);

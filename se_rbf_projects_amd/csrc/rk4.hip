@@ -563,6 +563,7 @@ void free_state(StateBuffers<T> &sb)
     for (T *b : {sb.qT, sb.qA, sb.qB, sb.F, sb.Wx, sb.Wy, sb.WL, sb.basis, sb.lev, sb.vW, sb.tscal, sb.gamma, sb.WL1,
                  sb.WL2})
         if (b) (void)hipFree(b);
+    if (sb.s16) (void)hipFree(sb.s16);
     sb = StateBuffers<T>();
 }
 
@@ -769,7 +770,15 @@ int enqueue_step(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off)
     a.F = sb.F;
     int rc;
     // the closure kernel runs at every step and re-evaluates on multiples of closure_every
-    if (e->closure && (rc = closure_enqueue<T>(e, sb, dt, step_off))) return rc;
+    // with the FP32 tile kernel the closure hands over its multiplier in FP16 (half the bytes
+    // of the per-point coefficient stream that every stage reads)
+    const bool compact = std::is_same<T, float>::value && e->closure && sb.s16 && e->kver == 5 && hv1 != 2 &&
+                         mtile_supported(e);
+    if (compact) {
+        a.hv_mode = 2;
+        a.s16 = sb.s16;
+    }
+    if (e->closure && (rc = closure_enqueue<T>(e, sb, dt, step_off, compact))) return rc;
     a.qin = sb.qT; a.qout = sb.qB;
     if ((rc = launch_stage<T>(e, 1, hv1, a))) return rc;
     a.qin = sb.qB; a.qout = sb.qA;

@@ -25,6 +25,7 @@ struct StageArgs {
     const int *base;    // step counter
     const T *uarr;      // VEL_ARRAY: Up [N][3][LANES]
     const T *gamma;     // per-point hyperviscosity coefficient (hv_mode 1)
+    const _Float16 *s16;  // per-point multiplier of hv_coef in FP16 (hv_mode 2; FP32 tile kernel)
     int N, n, Nv, nv;
     int flow, ref_mode, hv_mode;
     int step_off;

@@ -64,17 +64,22 @@ def traffic(fetch_dir, write_dir, copy_mib):
         if fk is None or wk is None:
             continue
         out[k] = {"fetch_KiB": fk, "write_KiB": wk, "hbm_bytes_per_launch": kf * fk * 1024.0 + kw * wk * 1024.0}
-    # one RK4 step = stage launches 1..4 with Delta^k (STAGE 0 is the right-hand-side evaluation)
+    # one RK4 step = stage launches 1..4 with Delta^k (STAGE 0 is the right-hand-side evaluation):
+    # mtile2_kernel<T, STAGE, HV, TPW> or stage2_kernel<T, S, NTR, STAGE, HV, U>
     for prec in ("float", "double"):
-        for fam in ("mtile2_kernel<%s, " % prec, "stage2_kernel<%s, 64, 4, " % prec):
-            st = [out[k] for k in out if k.startswith(fam) and k.endswith(", 1>" if fam.startswith("mtile2") else
-                                                                            ", 1, 8>") and
-                  not k.startswith(fam + "0")]
+        for fam, ist, ihv in (("mtile2_kernel", 1, 2), ("stage2_kernel", 3, 4)):
+            st = []
+            for k, val in out.items():
+                if not k.startswith(fam + "<" + prec + ","):
+                    continue
+                args = [x.strip() for x in k[k.index("<") + 1:k.rindex(">")].split(",")]
+                if len(args) > ihv and args[ihv] == "1" and args[ist] in ("1", "2", "3", "4"):
+                    st.append(val)
             if len(st) == 4:
                 key = ("fp32" if prec == "float" else "fp64") + " step (Delta^4 in every stage)"
-                if key not in out or fam.startswith("mtile2"):
+                if key not in out or fam == "mtile2_kernel":
                     out[key] = {"hbm_bytes_per_launch": sum(x["hbm_bytes_per_launch"] for x in st) / 4,
-                                "kernel": fam + "STAGE, ...>"}
+                                "kernel": fam + "<" + prec + ", STAGE, ...>"}
     return {"calibration": {"copy_bytes_each_way": nbytes, "fetch_factor": kf, "write_factor": kw},
             "kernels": out}
 
