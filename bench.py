@@ -93,7 +93,7 @@ def error_vs(q, qref):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64, help="timed steps (a multiple of the closure cadence)")
+    ap.add_argument("--steps", type=int, default=128, help="timed steps (a multiple of the closure cadence)")
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--nv", type=int, default=60)

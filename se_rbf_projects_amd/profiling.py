@@ -23,7 +23,40 @@ def _dbs(path):
     return sorted(glob.glob(os.path.join(path, "**", "*.db"), recursive=True))
 
 
+_MANGLED_ARGS = {"f": "float", "d": "double", "i": "int", "b": "bool"}
+
+
+def _demangle_simple(name):
+    """kernel<args> from an Itanium-mangled serbf kernel name that the profiler left mangled
+    (e.g. one with a _Float16 parameter): the nested name's last component and its simple
+    template arguments (types f/d/i/b and integer literals Li<n>E)."""
+    import re
+    m = re.match(r"_ZN5serbf(?:12_GLOBAL__N_1)?(\d+)", name)
+    if not m:
+        return name
+    k = m.end()
+    n = int(m.group(1))
+    base = name[k:k + n]
+    rest = name[k + n:]
+    if not rest.startswith("I"):
+        return base
+    args, i = [], 1
+    while i < len(rest) and rest[i] != "E":
+        if rest[i] == "L":
+            j = rest.index("E", i)
+            args.append(rest[i + 2:j])
+            i = j + 1
+        elif rest[i] in _MANGLED_ARGS:
+            args.append(_MANGLED_ARGS[rest[i]])
+            i += 1
+        else:
+            return base
+    return base + "<" + ", ".join(args) + ">"
+
+
 def _short(name):
+    if name.startswith("_Z"):
+        return _demangle_simple(name)
     name = name.replace("(anonymous namespace)::", "").replace("void ", "").replace("serbf::", "")
     return name.split("(")[0]
 
