@@ -224,8 +224,13 @@ __host__ __device__ inline int mtile2_lds_bytes(int up, bool hv)
 
 // TPW tracers per wavefront: 4 (4 wavefronts per work-group) or 2 (8 wavefronts; wavefront w
 // takes level block w % 4 of tracers 2 (w / 4), 2 (w / 4) + 1)
+// Occupancy: 8 wavefronts per SIMD in FP32 (64 registers, 4 work-groups per CU) and 4 in FP64
+// (128 registers with a 20-byte spill, 2 work-groups per CU).  At 655362 x 60 x 4 this took the
+// FP32 stage from 1.43 to 1.36 ms and the FP64 stage from 2.96 ms (3 wavefronts per SIMD, one
+// work-group per CU) to 2.41 ms.
 template <typename T, int STAGE, int HV, int TPW>
-__global__ void __launch_bounds__(256 * (4 / TPW)) mtile2_kernel(StageArgs<T> a, MTilePlan tp, int up)
+__global__ void __launch_bounds__(256 * (4 / TPW)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 4)))
+mtile2_kernel(StageArgs<T> a, MTilePlan tp, int up)
 {
     extern __shared__ __align__(16) unsigned char m2_lds[];
     typedef typename Acc4<T>::type VT;
@@ -665,9 +670,9 @@ int launch_mtile3(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTil
 }
 
 // tracers per wavefront of the version-5 kernel: 2 (8 wavefronts per work-group).  Measured at
-// 655362 x 60 x 4 with the batched update loads: FP32 1.44 ms per stage (4 tracers per
-// wavefront: 1.44 ms at 4 wavefronts per SIMD), FP64 3.08 ms (4 tracers: 3.57 ms, one
-// wavefront per SIMD for lack of registers)
+// 655362 x 60 x 4 with the batched update loads, before the occupancy caps: FP32 1.44 ms per
+// stage (4 tracers per wavefront: 1.44 ms), FP64 3.08 ms (4 tracers: 3.57 ms, one wavefront
+// per SIMD for lack of registers)
 template <typename T>
 constexpr int kTPW = 2;
 
