@@ -257,7 +257,7 @@ def main():
         "setup": {"knn_s": knn_s, "weights_s": w_s, "weight_solves_per_s": 2 * N / w_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "mtile2_kernel<float, STAGE, 1> (matrix-core tiles), average of the 4 launches "
+                     "kernel": "mtile2_kernel<float, STAGE, 1, 2> (matrix-core tiles), average of the 4 launches "
                                "of a step",
                      "bytes_per_launch": b32,
                      "note": "average FP32 stage launch of the f32 leg (HIP events); algorithmic bytes model "
