@@ -442,8 +442,9 @@ int launch_weights(serbf_engine *e, const WArgs &a, int blocks_per_cu)
     return SERBF_OK;
 }
 
-// Schedule variants of the persistent solve: the LDS class (smallest of 48/80/112 rows that
-// holds the largest system), the residual of the refinement step (re-evaluated from the
+// Schedule variants of the persistent solve: the LDS class (smallest of 48/64/72/80/112 rows
+// that holds the largest system: at m = 70 the 72-row class fits 3 wavefronts per CU where the
+// 80-row class fits 2), the residual of the refinement step (re-evaluated from the
 // coordinates, or read from a copy of the matrix kept in LDS, when it fits), and the work
 // order (node order, or largest systems first).
 struct WVariant {
@@ -452,27 +453,43 @@ struct WVariant {
 };
 constexpr WVariant kWVariants[4] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
 
+template <int MAXM>
+int class_bytes(int copy)
+{
+    return copy ? Lds<MAXM, true>::bytes : Lds<MAXM, false>::bytes;
+}
+
 bool variant_fits(int maxm, int copy)
 {
-    const int bytes = maxm <= 48 ? (copy ? Lds<48, true>::bytes : Lds<48, false>::bytes)
-                      : maxm <= 80 ? (copy ? Lds<80, true>::bytes : Lds<80, false>::bytes)
-                                   : (copy ? Lds<112, true>::bytes : Lds<112, false>::bytes);
+    const int bytes = maxm <= 48   ? class_bytes<48>(copy)
+                      : maxm <= 64 ? class_bytes<64>(copy)
+                      : maxm <= 72 ? class_bytes<72>(copy)
+                      : maxm <= 80 ? class_bytes<80>(copy)
+                                   : class_bytes<112>(copy);
     return bytes <= 160 * 1024;
+}
+
+template <int MAXM>
+int launch_in(serbf_engine *e, const WArgs &a, int copy)
+{
+    return copy ? launch_weights<MAXM, true>(e, a, 0) : launch_weights<MAXM, false>(e, a, 0);
 }
 
 int launch_class(serbf_engine *e, const WArgs &a, int maxm, int copy)
 {
-    if (maxm <= 48) return copy ? launch_weights<48, true>(e, a, 0) : launch_weights<48, false>(e, a, 0);
-    if (maxm <= 80) return copy ? launch_weights<80, true>(e, a, 0) : launch_weights<80, false>(e, a, 0);
-    return copy ? launch_weights<112, true>(e, a, 0) : launch_weights<112, false>(e, a, 0);
+    if (maxm <= 48) return launch_in<48>(e, a, copy);
+    if (maxm <= 64) return launch_in<64>(e, a, copy);
+    if (maxm <= 72) return launch_in<72>(e, a, copy);
+    if (maxm <= 80) return launch_in<80>(e, a, copy);
+    return launch_in<112>(e, a, copy);
 }
 
-// largest system size over the nodes and the order of the nodes by decreasing size
-int system_sizes(serbf_engine *e, int &maxm, std::vector<int> &by_size)
+// system size of every node, the largest one and the order of the nodes by decreasing size
+int system_sizes(serbf_engine *e, int &maxm, std::vector<int> &by_size, std::vector<int> &m)
 {
     const serbf_config &c = e->cfg;
     const int N = e->N;
-    std::vector<int> m(N);
+    m.assign(N, 0);
     if (e->h_nn.empty()) {
         const int M = c.mode == SERBF_MODE_REFERENCE ? 21 : (c.poly_deg + 1) * (c.poly_deg + 2) / 2;
         for (int i = 0; i < N; i++) m[i] = e->n + M;
@@ -510,22 +527,6 @@ WArgs make_args(serbf_engine *e)
     return a;
 }
 
-// one timed launch of variant v over `count` work items of `order` (device array)
-int run_variant(serbf_engine *e, int v, int maxm, const int *d_order, int count, float *ms)
-{
-    WArgs a = make_args(e);
-    a.order = d_order;
-    a.count = count;
-    SERBF_HIP(hipMemsetAsync(e->d_scratch, 0, sizeof(int), e->stream));
-    SERBF_HIP(hipEventRecord(e->ev0, e->stream));
-    int rc = launch_class(e, a, maxm, kWVariants[v].copy);
-    if (rc) return rc;
-    SERBF_HIP(hipEventRecord(e->ev1, e->stream));
-    SERBF_HIP(hipEventSynchronize(e->ev1));
-    SERBF_HIP(hipEventElapsedTime(ms, e->ev0, e->ev1));
-    return SERBF_OK;
-}
-
 int upload_order(serbf_engine *e, const std::vector<int> &order)
 {
     if (e->d_worder) (void)hipFree(e->d_worder);
@@ -536,13 +537,49 @@ int upload_order(serbf_engine *e, const std::vector<int> &order)
     return SERBF_OK;
 }
 
+constexpr int kClasses[5] = {48, 64, 72, 80, 112};
+
+// One timed run of variant v over the work items `items` (node ids, in the order given).  The
+// items are grouped by LDS class (the smallest class that holds the node's system), and each
+// non-empty group is one persistent launch of its class, largest class first: uneven stencils
+// then run their small systems at the occupancy of a small class.
+int run_variant(serbf_engine *e, int v, const std::vector<int> &m, const std::vector<int> &items, float *ms)
+{
+    std::vector<int> grouped;
+    grouped.reserve(items.size());
+    int start[6] = {0, 0, 0, 0, 0, 0};
+    for (int ci = 4; ci >= 0; ci--) {
+        start[ci] = (int)grouped.size();
+        const int lo = ci > 0 ? kClasses[ci - 1] : 0, hi = kClasses[ci];
+        for (int it : items)
+            if (m[it] > lo && m[it] <= hi) grouped.push_back(it);
+    }
+    int rc = upload_order(e, grouped);
+    if (rc) return rc;
+    SERBF_HIP(hipEventRecord(e->ev0, e->stream));
+    for (int ci = 4; ci >= 0; ci--) {
+        const int off = start[ci], cnt = (ci > 0 ? start[ci - 1] : (int)grouped.size()) - off;
+        if (cnt <= 0) continue;
+        WArgs a = make_args(e);
+        a.order = e->d_worder + off;
+        a.count = cnt;
+        const int copy = kWVariants[v].copy && variant_fits(kClasses[ci], 1) ? 1 : 0;
+        SERBF_HIP(hipMemsetAsync(e->d_scratch, 0, sizeof(int), e->stream));
+        if ((rc = launch_class(e, a, kClasses[ci], copy))) return rc;
+    }
+    SERBF_HIP(hipEventRecord(e->ev1, e->stream));
+    SERBF_HIP(hipEventSynchronize(e->ev1));
+    SERBF_HIP(hipEventElapsedTime(ms, e->ev0, e->ev1));
+    return SERBF_OK;
+}
+
 }  // namespace
 
 int weights_run(serbf_engine *e)
 {
     int maxm = 0;
-    std::vector<int> by_size;
-    int rc = system_sizes(e, maxm, by_size);
+    std::vector<int> by_size, m;
+    int rc = system_sizes(e, maxm, by_size, m);
     if (rc) return rc;
     SERBF_CHECK(maxm <= 112, SERBF_ERR_UNSUPPORTED, "weights: n + polynomial terms must be <= 112");
     rc = alloc_weights(e);
@@ -550,14 +587,11 @@ int weights_run(serbf_engine *e)
     rc = ensure_scratch(e, 64);
     if (rc) return rc;
     int v = e->wvariant;
-    if (v < 0 || v > 3 || !variant_fits(maxm, kWVariants[v].copy)) v = 0;
-    const int *d_order = nullptr;
-    if (kWVariants[v].sorted) {
-        if ((rc = upload_order(e, by_size))) return rc;
-        d_order = e->d_worder;
-    }
+    if (v < 0 || v > 3) v = 0;
+    std::vector<int> items(e->N);
+    for (int i = 0; i < e->N; i++) items[i] = kWVariants[v].sorted ? by_size[i] : i;
     float ms = 0.f;
-    if ((rc = run_variant(e, v, maxm, d_order, e->N, &ms))) return rc;
+    if ((rc = run_variant(e, v, m, items, &ms))) return rc;
     e->wstats_ms = ms;
     e->wstats_systems = (double)e->N * (e->cfg.hv_order > 0 ? 2.0 : 1.0);
     e->wstats_variant = v;
@@ -583,8 +617,8 @@ int weights_run(serbf_engine *e)
 int weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants)
 {
     int maxm = 0;
-    std::vector<int> by_size;
-    int rc = system_sizes(e, maxm, by_size);
+    std::vector<int> by_size, m;
+    int rc = system_sizes(e, maxm, by_size, m);
     if (rc) return rc;
     SERBF_CHECK(maxm <= 112, SERBF_ERR_UNSUPPORTED, "weights: n + polynomial terms must be <= 112");
     rc = alloc_weights(e);
@@ -598,20 +632,16 @@ int weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants
     for (int i = 0; i < e->N; i++) rank[by_size[i]] = i;
     std::vector<int> sorted_pick(pick);
     std::stable_sort(sorted_pick.begin(), sorted_pick.end(), [&](int x, int y) { return rank[x] < rank[y]; });
-    int *d_pick = nullptr, *d_sorted = nullptr;
-    SERBF_HIP(hipMalloc((void **)&d_pick, sample * sizeof(int)));
-    SERBF_HIP(hipMalloc((void **)&d_sorted, sample * sizeof(int)));
-    SERBF_HIP(hipMemcpyAsync(d_pick, pick.data(), sample * sizeof(int), hipMemcpyHostToDevice, e->stream));
-    SERBF_HIP(hipMemcpyAsync(d_sorted, sorted_pick.data(), sample * sizeof(int), hipMemcpyHostToDevice, e->stream));
     const double systems = (double)sample * (e->cfg.hv_order > 0 ? 2.0 : 1.0);
     int best = 0;
     float best_ms = 1e30f;
     for (int v = 0; v < 4; v++) {
         float ms = -1.f;
-        if (variant_fits(maxm, kWVariants[v].copy)) {
-            // one warm-up launch, then the timed one
-            rc = run_variant(e, v, maxm, kWVariants[v].sorted ? d_sorted : d_pick, sample, &ms);
-            if (!rc) rc = run_variant(e, v, maxm, kWVariants[v].sorted ? d_sorted : d_pick, sample, &ms);
+        // a copy variant keeps the copy in the classes it fits (it runs without it in the others)
+        if (!kWVariants[v].copy || variant_fits(kClasses[0], 1)) {
+            // one warm-up run, then the timed one
+            rc = run_variant(e, v, m, kWVariants[v].sorted ? sorted_pick : pick, &ms);
+            if (!rc) rc = run_variant(e, v, m, kWVariants[v].sorted ? sorted_pick : pick, &ms);
             if (rc) break;
             if (ms < best_ms) { best_ms = ms; best = v; }
         }
@@ -621,8 +651,6 @@ int weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants
             report[3 * v + 2] = ms > 0.f ? systems / (ms * 1e-3) : 0.0;
         }
     }
-    (void)hipFree(d_pick);
-    (void)hipFree(d_sorted);
     if (rc) return rc;
     if (nvariants) *nvariants = 4;
     e->wvariant = best;
