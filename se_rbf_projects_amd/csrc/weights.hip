@@ -442,9 +442,9 @@ int launch_weights(serbf_engine *e, const WArgs &a, int blocks_per_cu)
     return SERBF_OK;
 }
 
-// Schedule variants of the persistent solve: the LDS class (smallest of 48/64/72/80/112 rows
-// that holds the largest system: at m = 70 the 72-row class fits 3 wavefronts per CU where the
-// 80-row class fits 2), the residual of the refinement step (re-evaluated from the
+// Schedule variants of the persistent solve: the LDS class (smallest of 48/56/64/72/80/88/112
+// rows that holds the largest system: at m = 70 the 72-row class fits 3 wavefronts per CU where
+// the 80-row class fits 2), the residual of the refinement step (re-evaluated from the
 // coordinates, or read from a copy of the matrix kept in LDS, when it fits), and the work
 // order (node order, or largest systems first).
 struct WVariant {
@@ -462,9 +462,11 @@ int class_bytes(int copy)
 bool variant_fits(int maxm, int copy)
 {
     const int bytes = maxm <= 48   ? class_bytes<48>(copy)
+                      : maxm <= 56 ? class_bytes<56>(copy)
                       : maxm <= 64 ? class_bytes<64>(copy)
                       : maxm <= 72 ? class_bytes<72>(copy)
                       : maxm <= 80 ? class_bytes<80>(copy)
+                      : maxm <= 88 ? class_bytes<88>(copy)
                                    : class_bytes<112>(copy);
     return bytes <= 160 * 1024;
 }
@@ -478,9 +480,11 @@ int launch_in(serbf_engine *e, const WArgs &a, int copy)
 int launch_class(serbf_engine *e, const WArgs &a, int maxm, int copy)
 {
     if (maxm <= 48) return launch_in<48>(e, a, copy);
+    if (maxm <= 56) return launch_in<56>(e, a, copy);
     if (maxm <= 64) return launch_in<64>(e, a, copy);
     if (maxm <= 72) return launch_in<72>(e, a, copy);
     if (maxm <= 80) return launch_in<80>(e, a, copy);
+    if (maxm <= 88) return launch_in<88>(e, a, copy);
     return launch_in<112>(e, a, copy);
 }
 
@@ -537,7 +541,8 @@ int upload_order(serbf_engine *e, const std::vector<int> &order)
     return SERBF_OK;
 }
 
-constexpr int kClasses[5] = {48, 64, 72, 80, 112};
+constexpr int kNClasses = 7;
+constexpr int kClasses[kNClasses] = {48, 56, 64, 72, 80, 88, 112};
 
 // One timed run of variant v over the work items `items` (node ids, in the order given).  The
 // items are grouped by LDS class (the smallest class that holds the node's system), and each
@@ -547,8 +552,8 @@ int run_variant(serbf_engine *e, int v, const std::vector<int> &m, const std::ve
 {
     std::vector<int> grouped;
     grouped.reserve(items.size());
-    int start[6] = {0, 0, 0, 0, 0, 0};
-    for (int ci = 4; ci >= 0; ci--) {
+    int start[kNClasses] = {};
+    for (int ci = kNClasses - 1; ci >= 0; ci--) {
         start[ci] = (int)grouped.size();
         const int lo = ci > 0 ? kClasses[ci - 1] : 0, hi = kClasses[ci];
         for (int it : items)
@@ -557,7 +562,7 @@ int run_variant(serbf_engine *e, int v, const std::vector<int> &m, const std::ve
     int rc = upload_order(e, grouped);
     if (rc) return rc;
     SERBF_HIP(hipEventRecord(e->ev0, e->stream));
-    for (int ci = 4; ci >= 0; ci--) {
+    for (int ci = kNClasses - 1; ci >= 0; ci--) {
         const int off = start[ci], cnt = (ci > 0 ? start[ci - 1] : (int)grouped.size()) - off;
         if (cnt <= 0) continue;
         WArgs a = make_args(e);
