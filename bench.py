@@ -104,6 +104,8 @@ def main():
     ap.add_argument("--period", action="store_true", help="also run the full period and state the errors")
     ap.add_argument("--closure-every", type=int, default=128, help="steps between closure re-evaluations")
     ap.add_argument("--no-config-c", action="store_true", help="skip the uneven batched weight-solve leg")
+    ap.add_argument("--kernel-version", type=int, default=5,
+                    help="stage kernel: 5 matrix-core tiles (default), 2 column kernel, 4/6 tile variants")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not part of "
@@ -124,6 +126,7 @@ def main():
     log(f"nodes N={N} generated in {time.time() - t0:.1f}s")
 
     eng = E.Engine(0)
+    eng.set_kernel_version(args.kernel_version)
     eng.set_nodes(xyz)
     t = time.time()
     eng.knn(cfg.n)
@@ -222,7 +225,8 @@ def main():
         cpu = cpu_baseline(eng, cfg, xyz, q0, dt, args.cpu_steps)
 
     kern = "fp32 step (Delta^4 in every stage)"
-    traffic = load_traffic(kern)
+    # the committed counter summary is of the default stage kernel
+    traffic = load_traffic(kern) if args.kernel_version == 5 else None
     idx, _ = eng.stencils()
     dense_fl, useful_fl, umean = tile_flops(idx, 64, ntr)
     mfma_tf = dense_fl / (res["f32"][1] / 1e3) / 1e12
@@ -245,7 +249,11 @@ def main():
                    "nodes": N, "levels": cfg.Nv, "tracers": ntr, "stencil": cfg.n, "dt_s": dt,
                    "point_update": "one (node, level) advanced one RK4 step (all 4 tracers)",
                    "closure": closure_kind, "closure_every_steps": args.closure_every, "closure_s_max": s_max,
-                   "stage_kernel": "matrix-core tiles (kernel version 5) in both precisions"},
+                   "stage_kernel": {2: "column kernel (version 2)", 4: "matrix-core tiles staged in LDS (version 4)",
+                                    5: "matrix-core tiles (version 5)",
+                                    6: "matrix-core tile pipeline (version 6)"}.get(args.kernel_version,
+                                                                                    f"version {args.kernel_version}")
+                   + " in both precisions where it applies"},
         "fp64_plain": {"value": value["f64"], "ms_per_step": res["f64"][0] / args.steps,
                        "stage_us": res["f64"][1] * 1e3, "roofline_achieved_GBs": achieved64,
                        "roofline_frac": achieved64 / HBM_PEAK_GBS},
@@ -257,8 +265,9 @@ def main():
         "setup": {"knn_s": knn_s, "weights_s": w_s, "weight_solves_per_s": 2 * N / w_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "mtile2_kernel<float, STAGE, 1, 2> (matrix-core tiles), average of the 4 launches "
-                               "of a step",
+                     "kernel": ("mtile2_kernel<float, STAGE, 1, 2> (matrix-core tiles)" if args.kernel_version == 5
+                                else f"stage kernel version {args.kernel_version}")
+                               + ", average of the 4 launches of a step",
                      "bytes_per_launch": b32,
                      "note": "average FP32 stage launch of the f32 leg (HIP events); algorithmic bytes model "
                              "bench.stage_bytes; traffic = calibrated FETCH_SIZE + WRITE_SIZE per launch",
@@ -284,6 +293,7 @@ def config_c_weights(E, level):
     N = len(xyz)
     cfg = config_deform(N, Nv=60, n=84, deg=4, nv=7)
     eng = E.Engine(0)
+    eng.set_kernel_version(args.kernel_version)
     eng.set_nodes(xyz)
     eng.knn(84)
     _, D = eng.stencils()

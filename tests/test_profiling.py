@@ -1,0 +1,37 @@
+"""The rocprofv3 summary helpers: kernel-name shortening (including names the profiler leaves
+mangled) and the per-step aggregation of the calibrated HBM traffic."""
+from se_rbf_projects_amd import profiling as P
+
+
+def test_short_names():
+    assert P._short("void serbf::(anonymous namespace)::mtile2_kernel<float, 1, 1, 2>(serbf::StageArgs<float>, "
+                    "serbf::(anonymous namespace)::MTilePlan, int)") == "mtile2_kernel<float, 1, 1, 2>"
+    # a kernel with a _Float16 parameter comes back mangled
+    assert P._short("_ZN5serbf12_GLOBAL__N_112apply_kernelIfLi64EEEvNS0_8FeatArgsIT_EENS0_7NetViewEPS3_PDF16_S3_fii"
+                    ) == "apply_kernel<float, 64>"
+    assert P._short("_ZN5serbf13stage2_kernelIdLi64ELi4ELi2ELi1ELi8EEEvNS_9StageArgsIT_EEii") == \
+        "stage2_kernel<double, 64, 4, 2, 1, 8>"
+    assert P._short("stream_copy_kernel(float4 const*, float4*, unsigned long)") == "stream_copy_kernel"
+
+
+def test_step_aggregation(monkeypatch):
+    """Stages 1-4 with Delta^k of each precision average into one step entry; stage 0 (the
+    right-hand-side evaluation) and launches without Delta^k are left out."""
+    fetch = {"stream_copy_kernel": {"FETCH_SIZE": 1024.0}}
+    write = {"stream_copy_kernel": {"WRITE_SIZE": 2048.0}}
+    for st in range(5):
+        for prec, scale in (("float", 1.0), ("double", 2.0)):
+            k = f"mtile2_kernel<{prec}, {st}, 1, 2>"
+            fetch[k] = {"FETCH_SIZE": scale * 100.0 * (st + 1)}
+            write[k] = {"WRITE_SIZE": scale * 10.0}
+    fetch["mtile2_kernel<float, 2, 0, 2>"] = {"FETCH_SIZE": 1e6}
+    write["mtile2_kernel<float, 2, 0, 2>"] = {"WRITE_SIZE": 1e6}
+    monkeypatch.setattr(P, "counter_stats", lambda paths: fetch if paths == ["f"] else write)
+    d = P.traffic("f", "w", 2.0)   # the copy moves 2 MiB each way: factors 2 (fetch) and 1 (write)
+    kf, kw = d["calibration"]["fetch_factor"], d["calibration"]["write_factor"]
+    assert abs(kf - 2.0) < 1e-12 and abs(kw - 1.0) < 1e-12
+    f32 = d["kernels"]["fp32 step (Delta^4 in every stage)"]["hbm_bytes_per_launch"]
+    expect = sum(2.0 * 100.0 * (st + 1) * 1024 + 10.0 * 1024 for st in (1, 2, 3, 4)) / 4
+    assert abs(f32 - expect) < 1e-6
+    f64 = d["kernels"]["fp64 step (Delta^4 in every stage)"]["hbm_bytes_per_launch"]
+    assert abs(f64 - 2.0 * expect) < 1e-6
