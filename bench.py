@@ -273,7 +273,8 @@ def main():
                              "bench.stage_bytes; traffic = calibrated FETCH_SIZE + WRITE_SIZE per launch",
                      "mfma_view": {"dense_flops_per_launch": dense_fl, "useful_flops_per_launch": useful_fl,
                                    "mean_tile_union": umean, "achieved_TFLOPs": mfma_tf,
-                                   "peak_TFLOPs": MFMA_F32_PEAK_TFLOPS, "frac": mfma_tf / MFMA_F32_PEAK_TFLOPS}},
+                                   "peak_TFLOPs": MFMA_F32_PEAK_TFLOPS, "frac": mfma_tf / MFMA_F32_PEAK_TFLOPS}
+                     if args.kernel_version in (4, 5, 6) else None},
         "cpu_baseline": cpu,
         "config_c_weights": weights_c,
     }
