@@ -174,7 +174,7 @@ int serbf_set_hv_split(serbf_engine *e, int split);
  * from the learned closure; NULL returns to the constant hv_coef. */
 int serbf_set_gamma(serbf_engine *e, const double *gamma);
 /* Stage kernel variant: 5 (default) matrix-core tiles with the state read directly as the
- * matrix operand (FP32 and FP64, 4 tracers of 64 levels, config-mode solid-body or
+ * matrix operand (FP32 and FP64, 1 or 4 tracers of 64 levels, config-mode solid-body or
  * deformational flow, hyperviscosity in every stage; other cases use 2), 2 column-vector
  * form, 3 LDS-tiled (columns of 64 levels, 1 or 4 tracers; other shapes use 2), 4
  * matrix-core tiles with the state staged in LDS (FP32, the cases of 5), 6 version 5 as a

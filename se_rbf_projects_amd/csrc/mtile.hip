@@ -214,27 +214,27 @@ void launch_one(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp,
 constexpr int kOwnLD = 80;   // own-column stride: 8 + 64 + 8 levels
 
 template <typename T>
-__host__ __device__ inline int mtile2_lds_bytes(int up, bool hv)
+__host__ __device__ inline int mtile2_lds_bytes(int up, bool hv, int ntr)
 {
     const int wld = up + 1;
     const int wreg = 16 * wld * (hv ? 2 : 1);
-    const int oreg = kMTileNodes * 4 * kOwnLD;
+    const int oreg = kMTileNodes * ntr * kOwnLD;
     return (32 + (wreg > oreg ? wreg : oreg)) * (int)sizeof(T) + up * (int)sizeof(int);
 }
 
-// TPW tracers per wavefront: 4 (4 wavefronts per work-group) or 2 (8 wavefronts; wavefront w
-// takes level block w % 4 of tracers 2 (w / 4), 2 (w / 4) + 1)
+// NTR tracers (4 or 1) of 64 levels per node; TPW tracers per wavefront: NTR / TPW groups of
+// 4 wavefronts, wavefront w taking level block w % 4 of tracers TPW (w / 4) .. TPW (w / 4) + TPW - 1
 // Occupancy: 8 wavefronts per SIMD in FP32 (64 registers, 4 work-groups per CU) and 4 in FP64
 // (128 registers with a 20-byte spill, 2 work-groups per CU).  At 655362 x 60 x 4 this took the
 // FP32 stage from 1.43 to 1.36 ms and the FP64 stage from 2.96 ms (3 wavefronts per SIMD, one
 // work-group per CU) to 2.41 ms.
-template <typename T, int STAGE, int HV, int TPW>
-__global__ void __launch_bounds__(256 * (4 / TPW)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 4)))
+template <typename T, int STAGE, int HV, int NTR, int TPW>
+__global__ void __launch_bounds__(256 * (NTR / TPW)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 4)))
 mtile2_kernel(StageArgs<T> a, MTilePlan tp, int up)
 {
     extern __shared__ __align__(16) unsigned char m2_lds[];
     typedef typename Acc4<T>::type VT;
-    constexpr int S = 64, NTR = 4, NTH = 256 * (4 / TPW);
+    constexpr int S = 64, NTH = 256 * (NTR / TPW);
     const int wld = up + 1;
     const int wreg = 16 * wld * (HV ? 2 : 1), oreg = kMTileNodes * NTR * kOwnLD;
     T *sv = (T *)m2_lds;
@@ -676,24 +676,31 @@ int launch_mtile3(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTil
 template <typename T>
 constexpr int kTPW = 2;
 
-template <typename T, int STAGE, int HV>
-void launch_two(serbf_engine *e, const StageArgs<T> &a, const MTilePlan &tp, int up, int bytes)
+template <typename T, int STAGE, int HV, int NTR, int TPW>
+void launch_two_n(serbf_engine *e, const StageArgs<T> &a, const MTilePlan &tp, int up, int bytes)
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)mtile2_kernel<T, STAGE, HV, kTPW<T>>,
+        (void)hipFuncSetAttribute((const void *)mtile2_kernel<T, STAGE, HV, NTR, TPW>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((mtile2_kernel<T, STAGE, HV, kTPW<T>>), dim3(e->ntiles), dim3(256 * (4 / kTPW<T>)), bytes,
+    hipLaunchKernelGGL((mtile2_kernel<T, STAGE, HV, NTR, TPW>), dim3(e->ntiles), dim3(256 * (NTR / TPW)), bytes,
                        e->stream, a, tp, up);
+}
+
+template <typename T, int STAGE, int HV>
+void launch_two(serbf_engine *e, const StageArgs<T> &a, const MTilePlan &tp, int up, int bytes)
+{
+    if (e->ntr == 4) launch_two_n<T, STAGE, HV, 4, kTPW<T>>(e, a, tp, up, bytes);
+    else launch_two_n<T, STAGE, HV, 1, 1>(e, a, tp, up, bytes);
 }
 
 template <typename T>
 int launch_mtile2(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTilePlan &tp)
 {
     const int up = (e->tile_umax + 15) & ~15;
-    const int bytes = mtile2_lds_bytes<T>(up, hv != 0);
+    const int bytes = mtile2_lds_bytes<T>(up, hv != 0, e->ntr);
     SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "matrix-core stage kernel: tile union too large");
     switch (stage * 2 + (hv ? 1 : 0)) {
     case 0: launch_two<T, 0, 0>(e, a, tp, up, bytes); break;
@@ -713,12 +720,12 @@ int launch_mtile2(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTil
 
 }  // namespace
 
-// The matrix-core stage kernels serve 4 tracers of 64 levels, flows whose horizontal velocity
-// does not depend on the level (config mode), and tile plans whose unions fit (version 4: FP32;
-// version 5: FP32 and FP64).
+// The matrix-core stage kernels serve 4 tracers (version 5 also 1 tracer) of 64 levels, flows
+// whose horizontal velocity does not depend on the level (config mode), and tile plans whose
+// unions fit (version 4: FP32; versions 5 and 6: FP32 and FP64).
 bool mtile_supported(const serbf_engine *e)
 {
-    return e->ntr == 4 && e->S == 64 && e->cfg.mode == SERBF_MODE_CONFIG &&
+    return (e->ntr == 4 || (e->ntr == 1 && e->kver == 5)) && e->S == 64 && e->cfg.mode == SERBF_MODE_CONFIG &&
            (e->cfg.flow == SERBF_FLOW_DEFORM || e->cfg.flow == SERBF_FLOW_SOLID_BODY) && e->have_plan &&
            e->tile_umax <= kMTileUcap && (size_t)e->N * e->ntr * e->S < (size_t)INT_MAX;
 }

@@ -798,7 +798,7 @@ int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, doubl
     int rc = ensure_tscal<T>(e, sb, nsteps, dt, t0);
     if (rc) return rc;
     if (e->kver == 3 && tile_supported(e) && (rc = tile_plan(e))) return rc;
-    if (e->kver >= 4 && e->ntr == 4 && e->S == 64 && e->n <= 128 && (rc = tile_plan(e))) return rc;
+    if (e->kver >= 4 && (e->ntr == 4 || e->ntr == 1) && e->S == 64 && e->n <= 128 && (rc = tile_plan(e))) return rc;
     const int zero = 0;
     SERBF_HIP(hipMemcpyAsync(e->d_base, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
     // the graph of kGraphSteps steps reads the step counter and bumps it at the end
