@@ -294,7 +294,6 @@ def config_c_weights(E, level):
     N = len(xyz)
     cfg = config_deform(N, Nv=60, n=84, deg=4, nv=7)
     eng = E.Engine(0)
-    eng.set_kernel_version(args.kernel_version)
     eng.set_nodes(xyz)
     eng.knn(84)
     _, D = eng.stencils()
