@@ -1,18 +1,21 @@
-// Stage kernel on the matrix cores (gfx950): the horizontal stencil of a 16-node tile as a
-// dense product on v_mfma_f32_16x16x4f32.
+// Stage kernels on the matrix cores (gfx950): the horizontal stencil of a 16-node tile as a
+// dense product, W (16 nodes x U union columns) x Q (U x levels), on v_mfma_f32_16x16x4f32
+// (FP32) or v_mfma_f64_16x16x4f64 (FP64).
 //
-// A work-group owns one tile of 16 consecutive (space-filling-curve ordered) nodes.  The
-// tile plan (tile.hip) gives the union of their stencils (U <= 256 nodes) and, for every
-// stencil entry, its position in the union.  The work-group
-//   1. scatters the combined advection weights u_x' Dx + u_y' Dy of its 16 nodes (and the
-//      Delta^k weights when the stage applies hyperviscosity) into dense 16 x U blocks in LDS,
-//   2. for each tracer, stages the union's 64-level columns in LDS (each gathered column is
-//      read once per tile instead of once per stencil entry),
-//   3. lets wavefront w compute levels 16w..16w+15 of the 16 nodes as W (16 x U) x Q (U x 16)
-//      on the matrix cores, then applies the vertical stencil (from the node's own staged
-//      column), the velocity and the fused RK4 update on its accumulator tile.
+// A tile is 16 consecutive (space-filling-curve ordered) nodes.  The tile plan (tile.hip) gives
+// the union of their stencils (U <= 256 columns) and, for every stencil entry, its position in
+// the union.  Per stage a work-group scatters the combined advection weights u_x' Dx + u_y' Dy
+// of its 16 nodes (and the Delta^k weights when the stage applies hyperviscosity) into dense
+// 16 x U blocks in LDS, multiplies them by the union's columns on the matrix cores, and applies
+// the vertical stencil, the velocity and the fused RK4 update to the result.
+//   version 4 (mtile_kernel, FP32): stages each tracer's union columns in LDS first;
+//   version 5 (mtile2_kernel, the default): reads the union columns straight from memory as
+//             the B operand, each column once per tile;
+//   version 6 (mtile3_kernel): version 5 as a persistent pipeline of matrix and update
+//             wavefronts.
 // MFMA operand layout (16x16x4, lane l, c = l % 16, g = l / 16):
-//   A (16 x 4): A[c][g]     B (4 x 16): B[g][c]     D (16 x 16): D[4g + i][c], i = 0..3
+//   A (16 x 4): A[c][g]     B (4 x 16): B[g][c]
+//   D (16 x 16): FP32 D[4g + i][c], FP64 D[g + 4i][c], i = 0..3
 #include <hip/hip_runtime.h>
 
 #include <climits>
