@@ -18,6 +18,9 @@ Both precisions use the matrix-core tile stage kernel (kernel version 5).
 --period runs both the f64 and the f32+closure legs over the full 12-day period of the
 reversing flow and states their error against the exact solution q(T) = q(0), together with
 FP32 at the constant coefficient and at s_max times it.
+Secondary legs (reported beside the headline, not in it): the uneven batched weight solves at
+the same level (config C), and the plain FP64 method over a full period one level lower
+(163842 nodes x 60, one field of two Gaussian hills; config B).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--period]
 """
@@ -104,6 +107,7 @@ def main():
     ap.add_argument("--period", action="store_true", help="also run the full period and state the errors")
     ap.add_argument("--closure-every", type=int, default=128, help="steps between closure re-evaluations")
     ap.add_argument("--no-config-c", action="store_true", help="skip the uneven batched weight-solve leg")
+    ap.add_argument("--no-config-b", action="store_true", help="skip the FP64 full-period leg at one level lower")
     ap.add_argument("--kernel-version", type=int, default=5,
                     help="stage kernel: 5 matrix-core tiles (default), 2 column kernel, 4/6 tile variants")
     args = ap.parse_args()
@@ -219,6 +223,7 @@ def main():
     log(f"stream copy {args.copy_mb} MiB: {copy_ms:.3f} ms, {copy_gbs:.0f} GB/s (read + write)")
 
     weights_c = None if args.no_config_c else config_c_weights(E, args.level)
+    period_b = None if args.no_config_b else config_b_period(E, args.level - 1, args.kernel_version)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -277,6 +282,7 @@ def main():
                      if args.kernel_version in (4, 5, 6) else None},
         "cpu_baseline": cpu,
         "config_c_weights": weights_c,
+        "config_b_fp64_period": period_b,
     }
     print(json.dumps(out), flush=True)
     closure.close()
@@ -316,6 +322,43 @@ def config_c_weights(E, level):
            "fp64_tflops": flops / (ms * 1e-3) / 1e12}
     log(f"config C weights: {systems:.0f} systems in {ms:.1f} ms ({out['solves_per_s']:.3e}/s, variant {v}, "
         f"{out['fp64_tflops']:.2f} TFLOP/s FP64)")
+    return out
+
+
+def config_b_period(E, level, kernel_version):
+    """The plain FP64 method at scale: icosahedral level 7 (163842 nodes) x 60 levels, n = 55,
+    degree-4 tangent polynomials, vertical n = 7, Delta^4, the deformational flow, one field of
+    two Gaussian hills (seed 2024), run over the full 12-day period and compared with its
+    initial state (the exact solution of the reversing flow)."""
+    from se_rbf_projects_amd.ctypes_api import F64, config_deform, courant_dt
+    xyz = E.icos_nodes(level)
+    xyz = xyz[E.spatial_order(xyz)]
+    N = len(xyz)
+    cfg = config_deform(N, Nv=60, n=55, deg=4, nv=7)
+    eng = E.Engine(0)
+    eng.set_kernel_version(kernel_version)
+    eng.set_nodes(xyz)
+    eng.knn(cfg.n)
+    eng.configure(cfg)
+    eng.weights()
+    q0 = E.initial_condition(E.IC_HILLS, cfg, xyz, seed=2024)
+    nT = int(np.ceil(cfg.T / courant_dt(cfg, N)))
+    dt = cfg.T / nT
+    eng.set_state(q0)
+    eng.run(F64, 8, dt)
+    eng.set_state(q0)
+    ms, stage_ms = eng.run_timed(F64, nT, dt)
+    q = eng.state(F64)
+    eng.close()
+    d = q - q0
+    out = {"nodes": N, "levels": cfg.Nv, "tracers": 1, "steps": nT, "dt_s": dt, "ms_per_step": ms / nT,
+           "stage_us": stage_ms * 1e3, "point_updates_per_s": N * cfg.Nv * nT / (ms * 1e-3),
+           "period_rel_l2": float(np.linalg.norm(d) / np.linalg.norm(q0)),
+           "period_rel_linf": float(np.max(np.abs(d)) / np.max(np.abs(q0))),
+           "mass_rel": float((np.sum(q) - np.sum(q0)) / np.sum(q0))}
+    log(f"config B: {N} x {cfg.Nv} FP64, {nT} steps in {ms / 1e3:.2f} s ({ms / nT:.3f} ms/step, "
+        f"{out['point_updates_per_s']:.3e} point-updates/s), period rel l2 {out['period_rel_l2']:.4f}, "
+        f"linf {out['period_rel_linf']:.4f}")
     return out
 
 
