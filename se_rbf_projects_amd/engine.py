@@ -334,6 +334,9 @@ class Engine:
         _check(lib().serbf_set_gamma(self._e, _p(g)), "set_gamma")
 
     def set_kernel_version(self, v):
+        """Stage kernel: 5 (default) matrix-core tiles, FP32 and FP64, 1 or 4 tracers of 64
+        levels in config mode; 2 column kernel (every shape, and the fallback of the tile
+        kernels); 1, 3, 4, 6 other variants (see serbf.h)."""
         _check(lib().serbf_set_kernel_version(self._e, int(v)), "set_kernel_version")
 
     def set_tile_chunk(self, nbytes):
