@@ -648,11 +648,18 @@ int launch_mtile3(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTil
         hipDeviceProp_t prop;
         e->cus = hipGetDeviceProperties(&prop, e->device) == hipSuccess ? prop.multiProcessorCount : 256;
     }
-    // resident work-groups only (the kernel is persistent), a multiple of 8
-    int per_cu = 0;
-    SERBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)mtile3_kernel<T, 1, 1>, 768,
-                                                           bytes));
-    if (per_cu < 1) per_cu = 1;
+    // resident work-groups only (the kernel is persistent), a multiple of 8; the occupancy of
+    // the stage-1 instantiation (with its LDS limit raised first) stands for all stages, and is
+    // computed once per LDS size
+    static int per_cu = 0, per_cu_bytes = -1;
+    if (bytes != per_cu_bytes) {
+        (void)hipFuncSetAttribute((const void *)mtile3_kernel<T, 1, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        int pc = 0;
+        SERBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, (const void *)mtile3_kernel<T, 1, 1>, 768, bytes));
+        per_cu = pc < 1 ? 1 : pc;
+        per_cu_bytes = bytes;
+    }
     int grid = e->cus * per_cu;
     grid = grid / 8 * 8;
     if (grid < 8) grid = 8;
