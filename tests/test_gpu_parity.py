@@ -214,12 +214,18 @@ def test_stage_kernels_on_64_levels_match_oracle(eng, golden, kver, ntr):
         eng.run(F32, 12, dt)
         return q, eng.state(F32)
 
+    P = O.Problem(cfg, xyz, idx, Dx, Dy, L)
     try:
         q, q32 = run(kver)
         _, q32c = run(2)
+        # the right-hand side alone (stage 0 of the kernel) at two times of the flow
+        eng.set_kernel_version(kver)
+        eng.set_state(q0)
+        for t in (0.0, 0.37 * cfg.T):
+            assert relmax(eng.rhs(t), P.rhs(q0, t)) < 1e-11
     finally:
         eng.set_kernel_version(5)
-    oq = O.Problem(cfg, xyz, idx, Dx, Dy, L).rk4(q0, 12, dt)
+    oq = P.rk4(q0, 12, dt)
     assert rel_l2(q, oq) < 1e-11
     assert rel_l2(q32, oq) < 1e-4
     if kver != 2:
