@@ -29,7 +29,10 @@ namespace serbf {
 namespace {
 
 constexpr int kMTileNodes = 16;
-constexpr int kMTileUcap = 256;
+// largest tile union (columns) the matrix-core kernels take: 320 covers the n = 84 stencils of
+// uneven node sets (266 columns at most on jittered level-5 nodes); the LDS then stays within
+// 128 KB (version 4), 84 KB (version 5, FP64) and 147 KB (version 6, FP64)
+constexpr int kMTileUcap = 320;
 constexpr int kQLD = 68;        // staged column stride (64 levels + pad)
 
 typedef float v4f __attribute__((ext_vector_type(4)));

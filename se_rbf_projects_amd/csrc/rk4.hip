@@ -879,6 +879,8 @@ void rk4_free(serbf_engine *e)
 {
     free_state<double>(e->s64);
     free_state<float>(e->s32);
+    // the reference copy of the given state has the state's shape, which a reconfiguration changes
+    if (e->d_q0) { (void)hipFree(e->d_q0); e->d_q0 = nullptr; }
     if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
     e->tcap = 0;
 }
