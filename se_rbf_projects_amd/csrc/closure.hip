@@ -892,6 +892,9 @@ int serbf_closure_set_data(serbf_closure *c, long n, const float *feat, const fl
     SERBF_CHECK(c && n > 0 && feat && A && R, SERBF_ERR_ARG, "closure_set_data: bad arguments");
     SERBF_HIP(hipSetDevice(c->device));
     if (n > c->cap) {
+        // a failed allocation leaves no samples and no capacity, not a stale size
+        c->n = 0;
+        c->cap = 0;
         for (float **p : {&c->feat, &c->A, &c->R})
             if (*p) { (void)hipFree(*p); *p = nullptr; }
         SERBF_HIP(hipMalloc((void **)&c->feat, (size_t)n * 16 * sizeof(float)));
@@ -925,7 +928,8 @@ int serbf_closure_train(serbf_closure *c, long first, long count, int steps, int
                 "closure_train: the sample range must lie in the set and hold at least one batch");
     SERBF_HIP(hipSetDevice(c->device));
     if (steps > c->loss_cap) {
-        if (c->losses) (void)hipFree(c->losses);
+        if (c->losses) { (void)hipFree(c->losses); c->losses = nullptr; }
+        c->loss_cap = 0;
         SERBF_HIP(hipMalloc((void **)&c->losses, (size_t)steps * sizeof(float)));
         c->loss_cap = steps;
     }
@@ -948,7 +952,8 @@ int serbf_closure_eval(serbf_closure *c, long first, long count, float *o)
                 "closure_eval: sample range must lie in the set and be a multiple of 16");
     SERBF_HIP(hipSetDevice(c->device));
     if (count > c->out_cap) {
-        if (c->out) (void)hipFree(c->out);
+        if (c->out) { (void)hipFree(c->out); c->out = nullptr; }
+        c->out_cap = 0;
         SERBF_HIP(hipMalloc((void **)&c->out, (size_t)count * sizeof(float)));
         c->out_cap = count;
     }
@@ -972,11 +977,13 @@ int serbf_closure_features(serbf_engine *e, double t, double dt, float *feat, do
     double ts[4 * kTimeScalars] = {0};
     time_scalars_public(e->cfg, t, ts);
     const size_t nsm = (size_t)e->N * e->ntr * e->cfg.Nv;
-    double *d_ts = nullptr, *d_lap = nullptr;
-    float *d_feat = nullptr;
-    SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
-    SERBF_HIP(hipMalloc((void **)&d_lap, nsm * sizeof(double)));
-    SERBF_HIP(hipMalloc((void **)&d_feat, nsm * 16 * sizeof(float)));
+    DevTemp<double> dts, dlap;
+    DevTemp<float> dfeat;
+    SERBF_HIP(dts.alloc(4 * kTimeScalars));
+    SERBF_HIP(dlap.alloc(nsm));
+    SERBF_HIP(dfeat.alloc(nsm * 16));
+    double *d_ts = dts.p, *d_lap = dlap.p;
+    float *d_feat = dfeat.p;
     SERBF_HIP(hipMemcpyAsync(d_ts, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
     const FeatArgs<double> a = feat_args<double>(e, e->s64, e->s64.qT, dt, d_ts, e->d_base + 1, 0);
     const long ncol = (long)e->N * e->ntr;
@@ -984,16 +991,10 @@ int serbf_closure_features(serbf_engine *e, double t, double dt, float *feat, do
     const dim3 grid((unsigned)((waves + 3) / 4));
     if (e->S == 16) hipLaunchKernelGGL((features_kernel<double, 16>), grid, dim3(256), 0, e->stream, a, d_feat, d_lap);
     else hipLaunchKernelGGL((features_kernel<double, 64>), grid, dim3(256), 0, e->stream, a, d_feat, d_lap);
-    hipError_t le = hipGetLastError();
-    if (le == hipSuccess) {
-        (void)hipMemcpyAsync(feat, d_feat, nsm * 16 * sizeof(float), hipMemcpyDeviceToHost, e->stream);
-        (void)hipMemcpyAsync(lap, d_lap, nsm * sizeof(double), hipMemcpyDeviceToHost, e->stream);
-        le = hipStreamSynchronize(e->stream);
-    }
-    (void)hipFree(d_ts);
-    (void)hipFree(d_lap);
-    (void)hipFree(d_feat);
-    SERBF_HIP(le);
+    SERBF_HIP(hipGetLastError());
+    SERBF_HIP(hipMemcpyAsync(feat, d_feat, nsm * 16 * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    SERBF_HIP(hipMemcpyAsync(lap, d_lap, nsm * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
     return SERBF_OK;
 }
 
@@ -1037,10 +1038,10 @@ int serbf_closure_gamma(serbf_engine *e, int precision, double t, double dt, dou
     if (precision == SERBF_F32) {
         float tsf[4 * kTimeScalars];
         for (int i = 0; i < 4 * kTimeScalars; i++) tsf[i] = (float)ts[i];
-        float *d_ts = nullptr;
-        SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(tsf)));
-        SERBF_HIP(hipMemcpyAsync(d_ts, tsf, sizeof(tsf), hipMemcpyHostToDevice, e->stream));
-        const FeatArgs<float> a = feat_args<float>(e, e->s32, e->s32.qT, dt, d_ts, e->d_base + 1, 0);
+        DevTemp<float> d_ts;
+        SERBF_HIP(d_ts.alloc(4 * kTimeScalars));
+        SERBF_HIP(hipMemcpyAsync(d_ts.p, tsf, sizeof(tsf), hipMemcpyHostToDevice, e->stream));
+        const FeatArgs<float> a = feat_args<float>(e, e->s32, e->s32.qT, dt, d_ts.p, e->d_base + 1, 0);
         rc = launch_apply<float>(e, a, e->s32.gamma, 1);
         if (!rc) {
             std::vector<float> g(P);
@@ -1049,17 +1050,15 @@ int serbf_closure_gamma(serbf_engine *e, int precision, double t, double dt, dou
             for (size_t col = 0; col < cols; col++)
                 for (int v = 0; v < Nv; v++) gamma[col * Nv + v] = g[col * S + v];
         }
-        (void)hipFree(d_ts);
     } else {
-        double *d_ts = nullptr;
-        SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
-        SERBF_HIP(hipMemcpyAsync(d_ts, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
-        const FeatArgs<double> a = feat_args<double>(e, e->s64, e->s64.qT, dt, d_ts, e->d_base + 1, 0);
+        DevTemp<double> d_ts;
+        SERBF_HIP(d_ts.alloc(4 * kTimeScalars));
+        SERBF_HIP(hipMemcpyAsync(d_ts.p, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
+        const FeatArgs<double> a = feat_args<double>(e, e->s64, e->s64.qT, dt, d_ts.p, e->d_base + 1, 0);
         rc = launch_apply<double>(e, a, e->s64.gamma, 1);
         if (!rc) {
             rc = download_padded(e, gamma, e->s64.gamma, cols, Nv, S);
         }
-        (void)hipFree(d_ts);
     }
     return rc;
 }

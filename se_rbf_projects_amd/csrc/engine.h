@@ -32,6 +32,21 @@ void set_error(const std::string &msg);
         }                                                                                 \
     } while (0)
 
+// Device memory owned by one call: freed on every return path, early error returns included.
+// hipFree waits for the device, so work still queued on the buffer finishes first.
+template <typename T>
+struct DevTemp {
+    T *p = nullptr;
+    DevTemp() = default;
+    DevTemp(const DevTemp &) = delete;
+    DevTemp &operator=(const DevTemp &) = delete;
+    ~DevTemp()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t n) { return hipMalloc((void **)&p, n * sizeof(T)); }
+};
+
 // Largest vertical column handled by one wavefront (lane = level).
 constexpr int kMaxLevels = 64;
 // Largest vertical stencil.

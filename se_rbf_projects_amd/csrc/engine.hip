@@ -167,12 +167,13 @@ int serbf_set_nodes(serbf_engine *e, int N, const double *x, const double *y, co
     SERBF_HIP(hipSetDevice(e->device));
     free_nodes(e);
     e->configured = false;
-    e->N = N;
+    e->N = 0;   // no node set until every copy below has landed
     for (double **p : {&e->d_x, &e->d_y, &e->d_z}) SERBF_HIP(hipMalloc((void **)p, N * sizeof(double)));
     SERBF_HIP(hipMemcpyAsync(e->d_x, x, N * sizeof(double), hipMemcpyHostToDevice, e->stream));
     SERBF_HIP(hipMemcpyAsync(e->d_y, y, N * sizeof(double), hipMemcpyHostToDevice, e->stream));
     SERBF_HIP(hipMemcpyAsync(e->d_z, z, N * sizeof(double), hipMemcpyHostToDevice, e->stream));
     SERBF_HIP(hipStreamSynchronize(e->stream));
+    e->N = N;
     return SERBF_OK;
 }
 
@@ -181,6 +182,7 @@ int serbf_set_nodes(serbf_engine *e, int N, const double *x, const double *y, co
 int serbf_set_lonlat(serbf_engine *e, const double *lam, const double *phi)
 {
     SERBF_CHECK(e && e->N > 0 && lam && phi, SERBF_ERR_ARG, "set_lonlat: bad arguments");
+    SERBF_HIP(hipSetDevice(e->device));
     for (double **p : {&e->d_lam, &e->d_phi})
         if (!*p) SERBF_HIP(hipMalloc((void **)p, e->N * sizeof(double)));
     SERBF_HIP(hipMemcpyAsync(e->d_lam, lam, e->N * sizeof(double), hipMemcpyHostToDevice, e->stream));
@@ -210,17 +212,16 @@ static int alloc_stencils_impl(serbf_engine *e, int n)
 {
     e->have_plan = false;
     clear_sizes(e);
-    if (e->d_idx && e->n != n) {
-        (void)hipFree(e->d_idx);
-        (void)hipFree(e->d_D);
+    if (e->n != n) {
+        if (e->d_idx) (void)hipFree(e->d_idx);
+        if (e->d_D) (void)hipFree(e->d_D);
         e->d_idx = nullptr;
         e->d_D = nullptr;
     }
     e->n = n;
-    if (!e->d_idx) {
-        SERBF_HIP(hipMalloc((void **)&e->d_idx, (size_t)e->N * n * sizeof(int)));
-        SERBF_HIP(hipMalloc((void **)&e->d_D, (size_t)e->N * n * sizeof(double)));
-    }
+    // each buffer on its own, so a call after a failed allocation completes the pair
+    if (!e->d_idx) SERBF_HIP(hipMalloc((void **)&e->d_idx, (size_t)e->N * n * sizeof(int)));
+    if (!e->d_D) SERBF_HIP(hipMalloc((void **)&e->d_D, (size_t)e->N * n * sizeof(double)));
     return SERBF_OK;
 }
 
@@ -330,7 +331,7 @@ int serbf::alloc_weights(serbf_engine *e)
 {
     const size_t m = (size_t)e->N * e->n;
     for (double **p : {&e->d_Wx, &e->d_Wy, &e->d_WL, &e->d_WL1, &e->d_WL2}) {
-        if (*p) (void)hipFree(*p);
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
         SERBF_HIP(hipMalloc((void **)p, m * sizeof(double)));
     }
     if (!e->d_status) SERBF_HIP(hipMalloc((void **)&e->d_status, e->N * sizeof(int)));
@@ -506,9 +507,10 @@ int serbf_stream_copy(serbf_engine *e, size_t bytes, int reps, double *ms_per_co
     SERBF_CHECK(e && bytes >= 16 && reps >= 1 && ms_per_copy, SERBF_ERR_ARG, "stream_copy: bad arguments");
     SERBF_HIP(hipSetDevice(e->device));
     const size_t n16 = bytes / 16;
-    float4 *a = nullptr, *b = nullptr;
-    SERBF_HIP(hipMalloc((void **)&a, n16 * 16));
-    SERBF_HIP(hipMalloc((void **)&b, n16 * 16));
+    DevTemp<float4> da, db;
+    SERBF_HIP(da.alloc(n16));
+    SERBF_HIP(db.alloc(n16));
+    float4 *a = da.p, *b = db.p;
     SERBF_HIP(hipMemsetAsync(a, 0, n16 * 16, e->stream));
     const int blocks = 256 * 8 * 4;
     hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, e->stream, n16, a, b);
@@ -520,8 +522,6 @@ int serbf_stream_copy(serbf_engine *e, size_t bytes, int reps, double *ms_per_co
     float t = 0.f;
     SERBF_HIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
     *ms_per_copy = t / reps;
-    (void)hipFree(a);
-    (void)hipFree(b);
     return SERBF_OK;
 }
 
@@ -591,16 +591,13 @@ int serbf_norms(serbf_engine *e, int precision, const double *q_ref, double *out
     const int Nv = e->cfg.Nv, S = e->S;
     const size_t cols = (size_t)e->N * e->ntr;
     const double *ref = e->d_q0;
-    double *tmp = nullptr;
+    DevTemp<double> tmp;
     if (q_ref) {
-        SERBF_HIP(hipMalloc((void **)&tmp, cols * S * sizeof(double)));
-        SERBF_HIP(hipMemsetAsync(tmp, 0, cols * S * sizeof(double), e->stream));
-        const int urc = upload_padded(e, tmp, q_ref, cols, Nv, S);
-        if (urc) {
-            (void)hipFree(tmp);
-            return urc;
-        }
-        ref = tmp;
+        SERBF_HIP(tmp.alloc(cols * S));
+        SERBF_HIP(hipMemsetAsync(tmp.p, 0, cols * S * sizeof(double), e->stream));
+        const int urc = upload_padded(e, tmp.p, q_ref, cols, Nv, S);
+        if (urc) return urc;
+        ref = tmp.p;
     }
     const int blocks = 1024;
     int rc = ensure_scratch(e, blocks * 10 * sizeof(double));
@@ -616,7 +613,6 @@ int serbf_norms(serbf_engine *e, int precision, const double *q_ref, double *out
     std::vector<double> h(blocks * 10);
     SERBF_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     SERBF_HIP(hipStreamSynchronize(e->stream));
-    if (tmp) (void)hipFree(tmp);
     double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, -1e300, 1e300};
     for (int b = 0; b < blocks; b++) {
         for (int k = 0; k < 6; k++) a[k] += h[b * 10 + k];

@@ -851,8 +851,9 @@ int rk4_prepare(serbf_engine *e)
                 "rk4: configure, stencils and weights are required first");
     int rc = alloc_state<double>(e, e->s64);
     if (rc) return rc;
-    double *basis;
-    SERBF_HIP(hipMalloc((void **)&basis, (size_t)e->N * kBasis * sizeof(double)));
+    DevTemp<double> dbasis;
+    SERBF_HIP(dbasis.alloc((size_t)e->N * kBasis));
+    double *basis = dbasis.p;
     hipLaunchKernelGGL(basis_kernel, dim3((e->N + 255) / 256), dim3(256), 0, e->stream, e->N, e->d_x, e->d_y,
                        e->d_z, e->d_lam, e->d_phi, e->cfg, basis);
     SERBF_HIP(hipGetLastError());
@@ -862,7 +863,6 @@ int rk4_prepare(serbf_engine *e)
         if (!rc) rc = fill_tables<float>(e, e->s32, basis);
     }
     SERBF_HIP(hipStreamSynchronize(e->stream));
-    (void)hipFree(basis);
     if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
     e->f32_ready = rc == SERBF_OK;
     return rc;
@@ -890,15 +890,15 @@ int rk4_rhs(serbf_engine *e, double t, double *F_host)
     SERBF_CHECK(e->have_state, SERBF_ERR_STATE, "rhs: no state");
     double ts[4 * kTimeScalars];
     for (int k = 0; k < 4; k++) time_scalars(e->cfg, t, ts + k * kTimeScalars);
-    double *d_ts;
-    SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
-    SERBF_HIP(hipMemcpyAsync(d_ts, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
+    DevTemp<double> d_ts;
+    SERBF_HIP(d_ts.alloc(4 * kTimeScalars));
+    SERBF_HIP(hipMemcpyAsync(d_ts.p, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
     if (e->kver == 3 && tile_supported(e)) {
         const int prc = tile_plan(e);
         if (prc) return prc;
     }
     StageArgs<double> a = base_args<double>(e, e->s64, 0.0);
-    a.tscal = d_ts;
+    a.tscal = d_ts.p;
     a.base = e->d_base + 1;   // always zero
     a.qin = e->s64.qT;
     a.F = e->s64.qA;
@@ -907,7 +907,6 @@ int rk4_rhs(serbf_engine *e, double t, double *F_host)
         const size_t P = (size_t)e->N * e->ntr;
         rc = download_padded(e, F_host, e->s64.qA, P, e->cfg.Nv, e->S);
     }
-    (void)hipFree(d_ts);
     return rc;
 }
 
@@ -917,28 +916,25 @@ int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha
     SERBF_CHECK(e->s64.qT, SERBF_ERR_STATE, "eval: buffers not prepared");
     SERBF_CHECK(e->ntr == 1, SERBF_ERR_UNSUPPORTED, "eval: one tracer");
     const int Nv = e->cfg.Nv, S = e->S, N = e->N;
-    double *d_u;
-    SERBF_HIP(hipMalloc((void **)&d_u, (size_t)N * 3 * S * sizeof(double)));
-    SERBF_HIP(hipMemsetAsync(d_u, 0, (size_t)N * 3 * S * sizeof(double), e->stream));
-    int urc = upload_padded(e, d_u, Up_host, (size_t)N * 3, Nv, S);
+    DevTemp<double> d_u;
+    SERBF_HIP(d_u.alloc((size_t)N * 3 * S));
+    SERBF_HIP(hipMemsetAsync(d_u.p, 0, (size_t)N * 3 * S * sizeof(double), e->stream));
+    int urc = upload_padded(e, d_u.p, Up_host, (size_t)N * 3, Nv, S);
     if (!urc) urc = upload_padded(e, e->s64.qB, q_host, (size_t)N, Nv, S);
-    if (urc) {
-        (void)hipFree(d_u);
-        return urc;
-    }
+    if (urc) return urc;
     double ts[4 * kTimeScalars] = {0};
-    double *d_ts;
-    SERBF_HIP(hipMalloc((void **)&d_ts, sizeof(ts)));
-    SERBF_HIP(hipMemcpyAsync(d_ts, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
+    DevTemp<double> d_ts;
+    SERBF_HIP(d_ts.alloc(4 * kTimeScalars));
+    SERBF_HIP(hipMemcpyAsync(d_ts.p, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
     if (e->kver == 3 && tile_supported(e)) {
         const int prc = tile_plan(e);
         if (prc) return prc;
     }
     StageArgs<double> a = base_args<double>(e, e->s64, 0.0);
-    a.tscal = d_ts;
+    a.tscal = d_ts.p;
     a.base = e->d_base + 1;
     a.flow = VEL_ARRAY;
-    a.uarr = d_u;
+    a.uarr = d_u.p;
     a.alpha = alpha;
     a.qin = e->s64.qB;
     a.F = e->s64.qA;
@@ -946,8 +942,6 @@ int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha
     if (!rc) {
         rc = download_padded(e, F_host, e->s64.qA, (size_t)N, Nv, S);
     }
-    (void)hipFree(d_ts);
-    (void)hipFree(d_u);
     return rc;
 }
 

@@ -276,6 +276,35 @@ def test_weights_then_rk4_end_to_end(eng):
     assert np.max(np.abs(q - 1.0)) < 1e-9
 
 
+def test_reconfigure_to_a_larger_state(eng):
+    """A state of one shape, then a reconfiguration to a larger one (6x the levels, 4 tracers):
+    the copy of the given state that the norms compare against takes the new shape.  The norms
+    against it equal the norms against the same state passed explicitly, before and after a
+    run."""
+    xyz = E.icos_nodes(3)
+    N = len(xyz)
+    eng.set_nodes(xyz)
+    eng.knn(31)
+    small = config_deform(N, Nv=10, n=31, deg=3, nv=5)
+    eng.configure(small)
+    eng.weights()
+    eng.set_state(np.ones((N, 10)))
+    big = config_deform(N, Nv=60, n=31, deg=3, nv=5)
+    big.ntracers = 4
+    eng.configure(big)
+    eng.weights()
+    rng = np.random.default_rng(11)
+    q0 = 1.0 + 0.1 * rng.standard_normal((N, 4, 60))
+    eng.set_state(q0)
+    r0 = eng.norms(F64)
+    assert r0["L2"] == 0.0 and r0["Linf"] == 0.0
+    eng.run(F64, 3, courant_dt(big, N))
+    r1, r2 = eng.norms(F64), eng.norms(F64, q_ref=q0)
+    assert r1["L2"] > 0.0
+    for k in ("L1", "L2", "Linf", "mass_ref", "mass"):
+        assert r1[k] == r2[k], k
+
+
 @pytest.mark.parametrize("kver", [1, 2])
 def test_kernel_versions_agree_with_oracle(eng, probA, kver):
     cfg, xyz, idx, Dx, Dy, L = probA
