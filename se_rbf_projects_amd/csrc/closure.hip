@@ -820,12 +820,8 @@ using namespace serbf;
 
 extern "C" {
 
-int serbf_closure_create(serbf_closure **out, int device)
+static int closure_init(serbf_closure *c)
 {
-    SERBF_CHECK(out, SERBF_ERR_ARG, "closure_create: null");
-    SERBF_HIP(hipSetDevice(device));
-    serbf_closure *c = new serbf_closure();
-    c->device = device;
     SERBF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     SERBF_HIP(hipMalloc((void **)&c->params, kP * sizeof(float)));
     SERBF_HIP(hipMalloc((void **)&c->adam_m, kP * sizeof(float)));
@@ -836,6 +832,21 @@ int serbf_closure_create(serbf_closure **out, int device)
     SERBF_HIP(hipMalloc((void **)&c->norm, 32 * sizeof(float)));
     SERBF_HIP(hipMalloc((void **)&c->partials, (size_t)(kMaxWaves + 4) * kPS * sizeof(float)));
     SERBF_HIP(hipMalloc((void **)&c->grad, kPS * sizeof(float)));
+    return SERBF_OK;
+}
+
+int serbf_closure_create(serbf_closure **out, int device)
+{
+    SERBF_CHECK(out, SERBF_ERR_ARG, "closure_create: null");
+    *out = nullptr;
+    SERBF_HIP(hipSetDevice(device));
+    serbf_closure *c = new serbf_closure();
+    c->device = device;
+    const int rc = closure_init(c);
+    if (rc) {
+        serbf_closure_destroy(c);   // releases what was created before the failure
+        return rc;
+    }
     *out = c;
     return SERBF_OK;
 }
@@ -844,13 +855,13 @@ void serbf_closure_destroy(serbf_closure *c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (float *p : {c->params, c->adam_m, c->adam_v, c->norm, c->feat, c->A, c->R, c->partials, c->grad, c->losses,
                      c->out})
         if (p) (void)hipFree(p);
     for (unsigned short *p : {c->w1b, c->w2b, c->w2tb})
         if (p) (void)hipFree(p);
-    (void)hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 

@@ -103,20 +103,31 @@ extern "C" {
 
 const char *serbf_last_error(void) { return g_err.c_str(); }
 
+static int engine_init(serbf_engine *e)
+{
+    SERBF_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    SERBF_HIP(hipEventCreate(&e->ev0));
+    SERBF_HIP(hipEventCreate(&e->ev1));
+    SERBF_HIP(hipMalloc((void **)&e->d_base, 2 * sizeof(int)));
+    SERBF_HIP(hipMemset(e->d_base, 0, 2 * sizeof(int)));
+    return SERBF_OK;
+}
+
 int serbf_create(serbf_engine **out, int device)
 {
     SERBF_CHECK(out, SERBF_ERR_ARG, "create: null out");
+    *out = nullptr;
     int ndev = 0;
     SERBF_HIP(hipGetDeviceCount(&ndev));
     SERBF_CHECK(device >= 0 && device < ndev, SERBF_ERR_ARG, "create: no such device");
     SERBF_HIP(hipSetDevice(device));
     serbf_engine *e = new serbf_engine();
     e->device = device;
-    SERBF_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-    SERBF_HIP(hipEventCreate(&e->ev0));
-    SERBF_HIP(hipEventCreate(&e->ev1));
-    SERBF_HIP(hipMalloc((void **)&e->d_base, 2 * sizeof(int)));
-    SERBF_HIP(hipMemset(e->d_base, 0, 2 * sizeof(int)));
+    const int rc = engine_init(e);
+    if (rc) {
+        serbf_destroy(e);   // releases what was created before the failure
+        return rc;
+    }
     *out = e;
     return SERBF_OK;
 }
@@ -151,13 +162,13 @@ void serbf_destroy(serbf_engine *e)
 {
     if (!e) return;
     (void)hipSetDevice(e->device);
-    (void)hipStreamSynchronize(e->stream);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
     free_nodes(e);
     if (e->d_base) (void)hipFree(e->d_base);
     if (e->d_scratch) (void)hipFree(e->d_scratch);
-    (void)hipEventDestroy(e->ev0);
-    (void)hipEventDestroy(e->ev1);
-    (void)hipStreamDestroy(e->stream);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
 
@@ -277,6 +288,8 @@ int serbf_configure(serbf_engine *e, const serbf_config *cfg)
     }
     SERBF_HIP(hipSetDevice(e->device));
     rk4_free(e);
+    // a configuration that fails below leaves the engine unconfigured, not half-updated
+    e->configured = false;
     e->have_weights = false;
     e->have_state = false;
     e->cfg = c;

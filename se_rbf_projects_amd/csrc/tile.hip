@@ -426,9 +426,8 @@ int tile_plan(serbf_engine *e)
     SERBF_CHECK(e->n <= kTileKeys / kTileNodes, SERBF_ERR_UNSUPPORTED, "tile plan: n > 128");
     const int ntiles = (e->N + kTileNodes - 1) / kTileNodes;
     const int umax = kTileNodes * e->n < 1024 ? kTileNodes * e->n : 1024;
-    if (e->d_tuni) (void)hipFree(e->d_tuni);
-    if (e->d_tcount) (void)hipFree(e->d_tcount);
-    if (e->d_lidx) (void)hipFree(e->d_lidx);
+    for (void **p : {(void **)&e->d_tuni, (void **)&e->d_tcount, (void **)&e->d_lidx})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
     SERBF_HIP(hipMalloc((void **)&e->d_tuni, (size_t)ntiles * umax * sizeof(int)));
     SERBF_HIP(hipMalloc((void **)&e->d_tcount, (size_t)ntiles * sizeof(int)));
     SERBF_HIP(hipMalloc((void **)&e->d_lidx, (size_t)e->N * e->n * sizeof(unsigned short)));

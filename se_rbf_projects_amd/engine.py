@@ -161,7 +161,9 @@ class Closure:
     def set_params(self, params, norm):
         p = np.ascontiguousarray(params, np.float32)
         nm = np.ascontiguousarray(norm, np.float32)
-        assert p.size == self.NPARAMS and nm.size == 32
+        if p.size != self.NPARAMS or nm.size != 32:
+            raise ValueError(f"closure parameters: {self.NPARAMS} weights and 32 normalisation values expected, "
+                             f"got {p.size} and {nm.size}")
         _check(lib().serbf_closure_set_params(self._c, _p(p), _p(nm)), "closure_set_params")
 
     def set_smax(self, s_max):
@@ -177,7 +179,9 @@ class Closure:
         self._A = np.ascontiguousarray(A, np.float32)
         self._R = np.ascontiguousarray(R, np.float32)
         n = len(self._A)
-        assert self._feat.shape == (n, 16) and self._R.shape == (n,)
+        if self._feat.shape != (n, 16) or self._A.shape != (n,) or self._R.shape != (n,):
+            raise ValueError(f"closure data: feat (n, 16), A (n,), R (n,) expected, got {self._feat.shape}, "
+                             f"{self._A.shape}, {self._R.shape}")
         _check(lib().serbf_closure_set_data(self._c, n, _p(self._feat), _p(self._A), _p(self._R)), "closure_set_data")
         self.n = n
 
@@ -223,11 +227,18 @@ class Engine:
 
     def set_nodes(self, xyz, lonlat=None):
         xyz = _f64(xyz)
-        self.N = len(xyz)
-        self._xyz = [np.ascontiguousarray(xyz[:, k]) for k in range(3)]
-        _check(lib().serbf_set_nodes(self._e, self.N, *[_p(a) for a in self._xyz]), "set_nodes")
+        if xyz.ndim != 2 or xyz.shape[1] != 3 or len(xyz) == 0:
+            raise ValueError(f"set_nodes: (N, 3) coordinates expected, got {xyz.shape}")
         if lonlat is not None:
             lam, phi = (_f64(a) for a in lonlat)
+            if lam.size != len(xyz) or phi.size != len(xyz):
+                raise ValueError("set_nodes: lon/lat need one value per node")
+        self._xyz = [np.ascontiguousarray(xyz[:, k]) for k in range(3)]
+        self.N = 0
+        _check(lib().serbf_set_nodes(self._e, len(xyz), *[_p(a) for a in self._xyz]), "set_nodes")
+        self.N = len(xyz)
+        self.n = 0
+        if lonlat is not None:
             _check(lib().serbf_set_lonlat(self._e, _p(lam), _p(phi)), "set_lonlat")
 
     def knn(self, n):
@@ -242,13 +253,17 @@ class Engine:
 
     def set_stencils(self, idx):
         idx = np.ascontiguousarray(idx, dtype=np.int32)
+        if idx.ndim != 2 or idx.shape[0] != self.N:
+            raise ValueError(f"set_stencils: (N = {self.N}, n) indices expected, got {idx.shape}")
+        _check(lib().serbf_set_stencils(self._e, idx.shape[1], _p(idx)), "set_stencils")
         self.n = idx.shape[1]
-        _check(lib().serbf_set_stencils(self._e, self.n, _p(idx)), "set_stencils")
 
     def configure(self, cfg):
-        assert isinstance(cfg, SerbfConfig)
-        self.cfg = cfg
+        if not isinstance(cfg, SerbfConfig):
+            raise TypeError("configure: a SerbfConfig is expected")
+        self.cfg = None   # the library leaves the engine unconfigured when this fails
         _check(lib().serbf_configure(self._e, C.byref(cfg)), "configure")
+        self.cfg = cfg
 
     @property
     def ntr(self):
@@ -267,6 +282,8 @@ class Engine:
             return
         self._nn = np.ascontiguousarray(nn, np.int32)
         self._deg = np.ascontiguousarray(deg, np.int32)
+        if self._nn.size != self.N or self._deg.size != self.N:
+            raise ValueError("set_stencil_sizes: one size and one degree per node")
         _check(lib().serbf_set_stencil_sizes(self._e, _p(self._nn), _p(self._deg)), "set_stencil_sizes")
 
     def weights_autotune(self, sample=16384):
@@ -292,6 +309,9 @@ class Engine:
     def set_weights(self, Dx, Dy, L=None):
         Dx, Dy = _f64(Dx), _f64(Dy)
         L = None if L is None else _f64(L)
+        for w in (Dx, Dy) if L is None else (Dx, Dy, L):
+            if w.size != self.N * self.n:
+                raise ValueError(f"set_weights: N x n = {self.N} x {self.n} weights per operator expected, got {w.size}")
         _check(lib().serbf_set_weights(self._e, _p(Dx), _p(Dy), _p(L)), "set_weights")
 
     def vertical(self):
@@ -304,7 +324,8 @@ class Engine:
 
     def set_state(self, q):
         q = _f64(q)
-        assert q.size == self.N * self.ntr * self.cfg.Nv
+        if self.cfg is None or q.size != self.N * self.ntr * self.cfg.Nv:
+            raise ValueError("set_state: N x tracers x Nv values of a configured engine expected")
         _check(lib().serbf_set_state(self._e, _p(q)), "set_state")
 
     def state(self, precision=F64):
@@ -331,6 +352,8 @@ class Engine:
 
     def set_gamma(self, gamma):
         g = None if gamma is None else _f64(gamma)
+        if g is not None and (self.cfg is None or g.size != self.N * self.ntr * self.cfg.Nv):
+            raise ValueError("set_gamma: one coefficient per point (N x tracers x Nv) expected")
         _check(lib().serbf_set_gamma(self._e, _p(g)), "set_gamma")
 
     def set_kernel_version(self, v):
@@ -385,5 +408,7 @@ class Engine:
     def norms(self, precision=F64, q_ref=None):
         out = np.empty(7)
         qr = None if q_ref is None else _f64(q_ref)
+        if qr is not None and (self.cfg is None or qr.size != self.N * self.ntr * self.cfg.Nv):
+            raise ValueError("norms: the reference state needs N x tracers x Nv values")
         _check(lib().serbf_norms(self._e, int(precision), _p(qr), _p(out)), "norms")
         return dict(zip(("L1", "L2", "Linf", "mass_ref", "mass", "min", "max"), out))
