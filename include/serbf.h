@@ -117,6 +117,7 @@ int serbf_initial_condition(int kind, const serbf_config *cfg, int N, const doub
                             const double *z, unsigned long long seed, double *q);
 
 /* ---- engine ---- */
+/* *out is NULL on failure; nothing is left allocated. */
 int serbf_create(serbf_engine **out, int device);
 void serbf_destroy(serbf_engine *e);
 
@@ -129,6 +130,8 @@ int serbf_knn(serbf_engine *e, int n);
 int serbf_get_stencils(serbf_engine *e, int *idx, double *D);
 int serbf_set_stencils(serbf_engine *e, int n, const int *idx);
 
+/* Drops weights and state.  On failure the engine is left unconfigured (configure again before
+ * serbf_weights or serbf_set_state). */
 int serbf_configure(serbf_engine *e, const serbf_config *cfg);
 /* Batched FP64 weight solves for every node (one persistent launch). */
 int serbf_weights(serbf_engine *e);
@@ -215,6 +218,7 @@ int serbf_get_lap_weights(serbf_engine *e, double *L1, double *L2);
 #define SERBF_CLOSURE_HIDDEN 64
 #define SERBF_CLOSURE_PARAMS 4929
 typedef struct serbf_closure serbf_closure;
+/* *out is NULL on failure; nothing is left allocated. */
 int serbf_closure_create(serbf_closure **out, int device);
 void serbf_closure_destroy(serbf_closure *c);
 /* params float[SERBF_CLOSURE_PARAMS]; norm float[32] = shift[16], scale[16] (first 10 used).
