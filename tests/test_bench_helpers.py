@@ -27,3 +27,16 @@ def test_tile_flops_against_brute_force():
     assert useful == N * n * 64 * 4 * 2 * 2
     d1, u1, _ = bench.tile_flops(idx, S=64, ntr=4, hv=False)
     assert d1 == dense / 2 and u1 == useful / 2
+
+
+def test_more_than_one_gpu_is_refused():
+    """The bench measures one GPU; asked for more it exits with status 2 and says why, before
+    it loads the library or touches a device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
