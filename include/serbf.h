@@ -98,6 +98,13 @@ int serbf_icos_nodes(int level, double *x, double *y, double *z);
 /* Seeded tangential jitter: each node moves by U(0, frac*h) in a uniformly random
  * tangent direction and is renormalised; h = sqrt(8 pi / (sqrt(3) N)). */
 int serbf_jitter_nodes(int N, double *x, double *y, double *z, double frac, unsigned long long seed);
+/* Node-set file (the reference's nodesets/<name>.nc files, HDF5 as netCDF-4 writes it), read without
+ * netCDF: group "nodeset", float64 datasets x, y, z, lambda, phi.  Replaces the file reading of
+ * get_nodeset_from_file (nodesets.c:18-90).  With x, y, z, lambda and phi all NULL, sets *N to
+ * the node count only; otherwise *N is the capacity of the arrays on entry and the node count
+ * on return (x, y, z required; lambda, phi may be NULL).  Contiguous or compact datasets only;
+ * anything else is refused with SERBF_ERR_ARG and a message.  Host only. */
+int serbf_read_nodeset(const char *path, int *N, double *x, double *y, double *z, double *lambda, double *phi);
 /* Permutation perm[new] = old that orders nodes along a space-filling curve
  * (Hilbert order of the cube-face coordinates) for gather locality. */
 int serbf_spatial_order(int N, const double *x, const double *y, const double *z, int *perm);

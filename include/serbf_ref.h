@@ -145,6 +145,12 @@ typedef struct TC2_params {
 
 /* ---- Drop-in entry points (same names and signatures as the reference) ---- */
 
+/* nodesets.h / nodesets.c:18-90: reads a node-set file (x, y, z, lambda, phi of group
+ * "nodeset"), mallocs the five arrays into *nodeset, sets Nnodes and returns it.  Reads the HDF5
+ * file directly, without netCDF or MPI (one process).  A file it cannot read ends the process
+ * with a message, as the library's other drop-in entry points do. */
+int get_nodeset_from_file(char *nodeset_file, nodeset_struct *nodeset);
+
 /* nodesets.h:8 / nodesets.c:113-153: n nearest neighbours of every node by chord
  * distance, sorted by (d^2, index); allocates nodeset->idx and nodeset->D.
  * Runs the bucketed GPU kNN. */

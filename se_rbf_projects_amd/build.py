@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libserbf.so")
-SOURCES = ["engine.hip", "rk4.hip", "knn.hip", "weights.hip", "nodes.cpp", "dropin.hip", "tile.hip", "closure.hip", "mtile.hip"]
+SOURCES = ["engine.hip", "rk4.hip", "knn.hip", "weights.hip", "nodes.cpp", "nodefile.cpp", "dropin.hip", "tile.hip", "closure.hip", "mtile.hip"]
 ARCH = "gfx950"
 
 

@@ -211,6 +211,27 @@ extern "C" {
 
 void serbf_ref_last_verification(double out[7]) { std::memcpy(out, g_last_verification, sizeof(g_last_verification)); }
 
+/* nodesets.c:18-90 */
+int get_nodeset_from_file(char *nodeset_file, nodeset_struct *nodeset)
+{
+    int N = 0;
+    if (serbf_read_nodeset(nodeset_file, &N, nullptr, nullptr, nullptr, nullptr, nullptr)) fatal("get_nodeset_from_file");
+    double *a[5];
+    for (double *&p : a) {
+        p = (double *)std::malloc(sizeof(double) * (size_t)N);
+        if (!p) fatal("get_nodeset_from_file: malloc");
+    }
+    int n = N;
+    if (serbf_read_nodeset(nodeset_file, &n, a[0], a[1], a[2], a[3], a[4])) fatal("get_nodeset_from_file");
+    nodeset->Nnodes = N;
+    nodeset->x = a[0];
+    nodeset->y = a[1];
+    nodeset->z = a[2];
+    nodeset->lambda = a[3];
+    nodeset->phi = a[4];
+    return N;
+}
+
 /* nodesets.c:113-153 */
 void get_stencils(nodeset_struct *nodeset, int n)
 {

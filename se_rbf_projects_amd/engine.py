@@ -41,6 +41,7 @@ def lib():
             "serbf_icos_nodes": ([i, vp, vp, vp], i),
             "serbf_jitter_nodes": ([i, vp, vp, vp, d, C.c_ulonglong], i),
             "serbf_spatial_order": ([i, vp, vp, vp, vp], i),
+            "serbf_read_nodeset": ([C.c_char_p, C.POINTER(i), vp, vp, vp, vp, vp], i),
             "serbf_lonlat": ([i, vp, vp, vp, vp, vp], None),
             "serbf_initial_condition": ([i, vp, i, vp, vp, vp, C.c_ulonglong, vp], i),
             "serbf_create": ([C.POINTER(vp), i], i),
@@ -115,6 +116,18 @@ def icos_nodes(level):
     x, y, z = (np.empty(N) for _ in range(3))
     _check(lib().serbf_icos_nodes(level, _p(x), _p(y), _p(z)), "icos_nodes")
     return np.stack([x, y, z], axis=1)
+
+
+def read_nodeset(path):
+    """A node-set file of the reference (nodesets/<name>.nc), read without netCDF: float64
+    [N, 5] columns x, y, z, lambda, phi (nodesets.c:18-90 reads the same five datasets)."""
+    bpath = os.fsencode(path)
+    N = C.c_int(0)
+    _check(lib().serbf_read_nodeset(bpath, C.byref(N), None, None, None, None, None), "read_nodeset")
+    cols = [np.empty(N.value) for _ in range(5)]
+    n = C.c_int(N.value)
+    _check(lib().serbf_read_nodeset(bpath, C.byref(n), *[_p(c) for c in cols]), "read_nodeset")
+    return np.stack(cols, axis=1)
 
 
 def jitter_nodes(xyz, frac, seed):

@@ -14,12 +14,17 @@ Writes into tests/golden/:
   md25.npy            float64 [25, 4] columns x y z w of nodesets/md.00025
   md25_knn10.npy      int32 [25, 10]: the stencil table the reference prototype
                       printed for MD_25 with n = 10 (Advection/Python/Solver.ipynb:228-252)
+  ICOS_42.nc, ICOS_642.nc, MD_25.nc
+                      three of the reference's node files, copied unchanged, as inputs of
+                      the node-file reader's tests
+  md25_nc.npy         float64 [25, 5] (x, y, z, lambda, phi) of nodesets/MD_25.nc
 
 Only data is read (HDF5 datasets, a text table, and one notebook output cell
 parsed as JSON); no reference code is imported or executed.
 """
 import json
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -82,6 +87,9 @@ def main():
     tab = knn_table_from_notebook(os.path.join(ref, "Advection", "Python", "Solver.ipynb"))
     assert tab.shape == (25, 10)
     np.save(os.path.join(HERE, "md25_knn10.npy"), tab)
+    for name in ("ICOS_42.nc", "ICOS_642.nc", "MD_25.nc"):
+        shutil.copyfile(os.path.join(ns, name), os.path.join(HERE, name))
+    np.save(os.path.join(HERE, "md25_nc.npy"), read_nc(os.path.join(ns, "MD_25.nc")))
     print("wrote fixtures to", HERE)
 
 
