@@ -260,8 +260,11 @@ bool read_nodes(H5File &f, const char *path, int &N, double *const out[5], int c
     if (sv == 0 || sv == 1) {
         if (!f.have(0, 96)) return f.fail("superblock truncated");
         if (f.b[13] != 8 || f.b[14] != 8) return f.fail("offsets or lengths are not 8 bytes");
-        f.base = f.u(24, 8);
-        const uint64_t entry = sv == 0 ? 56 : 60;   // root group symbol table entry
+        // version 1 inserts the indexed-storage K (2 bytes) and 2 reserved bytes before the
+        // addresses: base address, free space, end of file, driver block, then the root entry
+        const uint64_t addrs = sv == 0 ? 24 : 28;
+        f.base = f.u(addrs, 8);
+        const uint64_t entry = addrs + 32;   // root group symbol table entry
         root = f.u(entry + 8, 8);
     } else if (sv == 2 || sv == 3) {
         if (!f.have(0, 48)) return f.fail("superblock truncated");

@@ -178,3 +178,39 @@ int main(int argc, char **argv)
     ref = np.load(gold("icos_642.npy"))
     assert int(N) == 642 and int(n2) == 642
     assert abs(float(s) - 642.0) < 1e-12 and float(phi_last) == ref[-1, 4]
+
+
+def test_superblock_versions_and_base_address(tmp_path):
+    """ICOS_42.nc (superblock version 0) rewritten with a version 1, 2 or 3 superblock and, for
+    some, a base address of 4 (all addresses in the file are relative to it): every variant
+    reads the same values.  The version 0 header keeps base, free-space, end-of-file and driver
+    addresses at byte 24 and the root entry at 56; version 1 moves them 4 bytes on; versions 2
+    and 3 keep base at 12 and the root object header address at 36."""
+    import struct
+    data = open(gold("ICOS_42.nc"), "rb").read()
+    ref = np.load(gold("icos_42.npy"))
+    assert data[8] == 0
+    undef = 0xFFFFFFFFFFFFFFFF
+    root = struct.unpack_from("<Q", data, 64)[0]
+    eof = struct.unpack_from("<Q", data, 40)[0]
+    body = data[96:]                                   # object headers and data, addresses from 96 on
+
+    def v1(base):
+        sb = bytearray(data[:24]) + struct.pack("<HH", 32, 0)      # indexed storage K, reserved
+        sb[8] = 1
+        sb += struct.pack("<4Q", base, undef, eof, undef)
+        sb += data[56:96]                                           # root group symbol table entry
+        assert len(sb) == 100
+        pad = bytes(base + 96 - len(sb)) if base + 96 > len(sb) else b""
+        assert base + 96 == len(sb) + len(pad)
+        return bytes(sb) + pad + body
+
+    def v23(ver, base):
+        sb = b"\x89HDF\r\n\x1a\n" + bytes([ver, 8, 8, 0]) + struct.pack("<4Q", base, undef, eof, root) + b"\0" * 4
+        assert len(sb) == 48
+        return sb + bytes(base + 96 - 48) + body
+
+    for name, blob in (("v1_base4", v1(4)), ("v2", v23(2, 0)), ("v2_base4", v23(2, 4)), ("v3_base4", v23(3, 4))):
+        p = tmp_path / (name + ".nc")
+        p.write_bytes(blob)
+        assert np.array_equal(E.read_nodeset(str(p)), ref), name
