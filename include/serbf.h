@@ -102,7 +102,7 @@ int serbf_jitter_nodes(int N, double *x, double *y, double *z, double frac, unsi
  * netCDF: group "nodeset", float64 datasets x, y, z, lambda, phi.  Replaces the file reading of
  * get_nodeset_from_file (nodesets.c:18-90).  With x, y, z, lambda and phi all NULL, sets *N to
  * the node count only; otherwise *N is the capacity of the arrays on entry and the node count
- * on return (x, y, z required; lambda, phi may be NULL).  Contiguous or compact datasets only;
+ * on return (x, y, z required; lambda, phi may be NULL).  Contiguous datasets only;
  * anything else is refused with SERBF_ERR_ARG and a message.  Host only. */
 int serbf_read_nodeset(const char *path, int *N, double *x, double *y, double *z, double *lambda, double *phi);
 /* Permutation perm[new] = old that orders nodes along a space-filling curve

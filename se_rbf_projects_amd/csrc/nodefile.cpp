@@ -9,9 +9,9 @@
 //   - version-2 object headers ("OHDR", continuation blocks "OCHK");
 //   - groups whose links are stored in the object header (link messages), hard links only;
 //   - datasets with a version 1 or 2 dataspace of rank 1, a little-endian IEEE float64
-//     datatype, and contiguous or compact storage (layout message version 3 or 4).
-// Chunked or filtered datasets, version-1 object headers, symbol-table groups and dense
-// link storage are reported as unsupported.  Every read is bounds-checked against the file
+//     datatype, and contiguous storage (layout message version 3 or 4).
+// Compact, chunked or filtered datasets, version-1 object headers, symbol-table groups and
+// dense link storage are reported as unsupported.  Every read is bounds-checked against the file
 // size, so a truncated or damaged file gives an error, not a wild read.
 #include <cstdint>
 #include <cstdio>
@@ -172,8 +172,7 @@ bool read_f64(H5File &f, uint64_t addr, const char *name, uint64_t &n, double *o
     std::vector<Message> msgs;
     if (!read_header(f, addr, msgs)) return false;
     bool space = false, dtype = false, layout = false;
-    uint64_t dim = 0, daddr = kUndef, dsize = 0, doff = 0;
-    bool compact = false;
+    uint64_t dim = 0, daddr = kUndef, dsize = 0;
     const std::string who = std::string("dataset '") + name + "': ";
     for (const Message &m : msgs) {
         const uint64_t o = m.off;
@@ -202,19 +201,10 @@ bool read_f64(H5File &f, uint64_t addr, const char *name, uint64_t &n, double *o
             if (m.size < 2) return f.fail(who + "layout truncated");
             const int ver = f.b[o], cls = f.b[o + 1];
             if (ver != 3 && ver != 4) return f.fail(who + "layout version is not 3 or 4");
-            if (cls == 1) {
-                if (m.size < 18) return f.fail(who + "layout truncated");
-                daddr = f.u(o + 2, 8);
-                dsize = f.u(o + 10, 8);
-            } else if (cls == 0) {
-                if (m.size < 4) return f.fail(who + "layout truncated");
-                dsize = f.u(o + 2, 2);
-                if (dsize > m.size - 4) return f.fail(who + "compact data overruns its message");
-                doff = o + 4;
-                compact = true;
-            } else {
-                return f.fail(who + "chunked or virtual storage is not supported");
-            }
+            if (cls != 1) return f.fail(who + "only contiguous storage is supported");
+            if (m.size < 18) return f.fail(who + "layout truncated");
+            daddr = f.u(o + 2, 8);
+            dsize = f.u(o + 10, 8);
             layout = true;
         } else if (m.type == 0x0b) {
             return f.fail(who + "filtered (compressed) datasets are not supported");
@@ -226,11 +216,9 @@ bool read_f64(H5File &f, uint64_t addr, const char *name, uint64_t &n, double *o
     n = dim;
     if (!out) return true;
     if (dim > cap) return f.fail(who + "more values than the output holds");
-    if (!compact) {
-        if (dim == 0) return true;
-        if (daddr == kUndef || daddr > ~0ull - f.base) return f.fail(who + "data never written");
-        doff = daddr + f.base;
-    }
+    if (dim == 0) return true;
+    if (daddr == kUndef || daddr > ~0ull - f.base) return f.fail(who + "data never written");
+    const uint64_t doff = daddr + f.base;
     if (!f.have(doff, dim * 8)) return f.fail(who + "data outside the file");
     for (uint64_t i = 0; i < dim; i++) {
         const uint64_t bits = f.u(doff + 8 * i, 8);
