@@ -3,8 +3,8 @@
 // (FP32) or v_mfma_f64_16x16x4f64 (FP64).
 //
 // A tile is 16 consecutive (space-filling-curve ordered) nodes.  The tile plan (tile.hip) gives
-// the union of their stencils (U <= 256 columns) and, for every stencil entry, its position in
-// the union.  Per stage a work-group scatters the combined advection weights u_x' Dx + u_y' Dy
+// the union of their stencils (U <= kMTileUcap = 320 columns; a stencil set with a wider tile
+// runs on the column kernel) and, for every stencil entry, its position in the union.  Per stage a work-group scatters the combined advection weights u_x' Dx + u_y' Dy
 // of its 16 nodes (and the Delta^k weights when the stage applies hyperviscosity) into dense
 // 16 x U blocks in LDS, multiplies them by the union's columns on the matrix cores, and applies
 // the vertical stencil, the velocity and the fused RK4 update to the result.
