@@ -354,6 +354,25 @@ class Engine:
     def run(self, precision, nsteps, dt, t0=0.0):
         _check(lib().serbf_run(self._e, int(precision), int(nsteps), float(dt), float(t0)), "run")
 
+    def run_history(self, precision, nsteps, dt, every, t0=0.0):
+        """RK4 steps with a history, as the reference prototype records q_hist
+        (PY/Solver.ipynb:669-773): the state at t0 and after every `every` steps.  Steps past the
+        last multiple of `every` are run but not recorded.  Returns times [H] and the states
+        [H, N, (tracers,) Nv]; the stepping between snapshots stays on the device."""
+        if int(every) < 1 or int(nsteps) < 0:
+            raise ValueError("run_history: every >= 1 and nsteps >= 0")
+        every, nsteps = int(every), int(nsteps)
+        times, hist = [float(t0)], [self.state(precision)]
+        done = 0
+        while done + every <= nsteps:
+            self.run(precision, every, dt, t0 + done * dt)
+            done += every
+            times.append(t0 + done * dt)
+            hist.append(self.state(precision))
+        if done < nsteps:
+            self.run(precision, nsteps - done, dt, t0 + done * dt)
+        return np.array(times), np.stack(hist)
+
     def run_timed(self, precision, nsteps, dt, t0=0.0):
         ms, st = C.c_double(), C.c_double()
         _check(lib().serbf_run_timed(self._e, int(precision), int(nsteps), float(dt), float(t0), C.byref(ms),

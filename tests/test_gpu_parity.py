@@ -305,6 +305,33 @@ def test_reconfigure_to_a_larger_state(eng):
         assert r1[k] == r2[k], k
 
 
+def test_run_history_snapshots(eng):
+    """run_history records the state at t0 and after every 2 steps of 7 (the 7th step runs but
+    is not recorded); each snapshot equals a fresh run of the same length from the same state,
+    to rounding of the time argument."""
+    xyz = E.icos_nodes(3)
+    N = len(xyz)
+    cfg = config_deform(N, Nv=10, n=31, deg=3, nv=5)
+    eng.set_nodes(xyz)
+    eng.knn(cfg.n)
+    eng.configure(cfg)
+    eng.weights()
+    q0 = O.initial(E.IC_HILLS, cfg, xyz, seed=4)
+    dt = courant_dt(cfg, N)
+    eng.set_state(q0)
+    times, hist = eng.run_history(F64, 7, dt, 2, t0=0.5 * dt)
+    assert hist.shape == (4, N, 10) and np.allclose(times, 0.5 * dt + dt * np.array([0, 2, 4, 6]))
+    assert np.array_equal(hist[0], q0)
+    last = eng.state(F64)
+    for k in (1, 2, 3):
+        eng.set_state(q0)
+        eng.run(F64, 2 * k, dt, 0.5 * dt)
+        assert rel_l2(hist[k], eng.state(F64)) < 1e-13, k
+    eng.set_state(q0)
+    eng.run(F64, 7, dt, 0.5 * dt)
+    assert rel_l2(last, eng.state(F64)) < 1e-13
+
+
 @pytest.mark.parametrize("kver", [1, 2])
 def test_kernel_versions_agree_with_oracle(eng, probA, kver):
     cfg, xyz, idx, Dx, Dy, L = probA
