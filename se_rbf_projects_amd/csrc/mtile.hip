@@ -212,11 +212,12 @@ void launch_one(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp,
 // together read each union column once per tile (the stencil-by-stencil gather of the column
 // kernel reads it once per stencil entry, about 5.5 times as often).  Nothing of the state is
 // staged in LDS, so the main loop has no barrier; LDS holds only the dense weight blocks and the
-// union offsets (about 34 KB at 256 union columns), which leaves room for four or five
-// work-groups per CU.  Each step of 16 union columns issues 16 independent loads (4 column
-// groups x 4 tracers) and 32 MFMAs into 8 independent accumulators (4 tracers x advection and
-// Delta^k).  After the loop the LDS is reused for the tile's own columns (zero-padded by 8
-// levels on either side) for the vertical stencil and the RK4 update.
+// union offsets (about 34 KB at 256 union columns), which leaves room for four work-groups per
+// CU.  With TPW tracers per wavefront, each step of 16 union columns issues 4 x TPW independent
+// loads (4 column groups x TPW tracers) and 8 x TPW MFMAs into 2 x TPW independent accumulators
+// (TPW tracers x advection and Delta^k): 8 loads and 16 MFMAs at TPW = 2.  After the loop the
+// LDS is reused for the tile's own columns (zero-padded by 8 levels on either side) for the
+// vertical stencil and the RK4 update.
 constexpr int kOwnLD = 80;   // own-column stride: 8 + 64 + 8 levels
 
 template <typename T>
