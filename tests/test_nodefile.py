@@ -144,8 +144,9 @@ def test_reader_under_address_sanitizer(tmp_path):
 
 
 def test_c_driver_reads_a_node_file(tmp_path):
-    """A C driver as a reference user writes it (wrappers.c:282): the drop-in header compiles
-    as C, links against libserbf.so, and get_nodeset_from_file fills the reference's struct."""
+    """A C driver as a reference user writes it (wrappers.c:282): both public headers compile
+    together as C11, the program links against libserbf.so, get_nodeset_from_file fills the
+    reference's struct, and the engine API's serbf_read_nodeset agrees with it."""
     import shutil
     import subprocess
     if shutil.which("gcc") is None:
@@ -155,13 +156,22 @@ def test_c_driver_reads_a_node_file(tmp_path):
     src.write_text(r'''
 #include <stdio.h>
 #include <stdlib.h>
+#include "serbf.h"
 #include "serbf_ref.h"
 int main(int argc, char **argv)
 {
+    int n = 0;
+    if (serbf_read_nodeset(argv[1], &n, NULL, NULL, NULL, NULL, NULL) != SERBF_OK) return 3;
+    double *zz = malloc(sizeof(double) * (size_t)n), *xx = malloc(sizeof(double) * (size_t)n),
+           *yy = malloc(sizeof(double) * (size_t)n);
+    if (serbf_read_nodeset(argv[1], &n, xx, yy, zz, NULL, NULL) != SERBF_OK) return 4;
     global_domains->Nnodes = get_nodeset_from_file(argv[1], global_domains->global_nodeset);
     const nodeset_struct *ns = global_domains->global_nodeset;
     double s = 0.0;
     for (int i = 0; i < ns->Nnodes; i++) s += ns->x[i] * ns->x[i] + ns->y[i] * ns->y[i] + ns->z[i] * ns->z[i];
+    for (int i = 0; i < n; i++)
+        if (xx[i] != ns->x[i] || yy[i] != ns->y[i] || zz[i] != ns->z[i]) return 5;
+    free(xx); free(yy); free(zz);
     printf("%d %d %.17g %.17g\n", global_domains->Nnodes, ns->Nnodes, s, ns->phi[ns->Nnodes - 1]);
     free(ns->x); free(ns->y); free(ns->z); free(ns->lambda); free(ns->phi);
     return 0;
