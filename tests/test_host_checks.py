@@ -81,3 +81,11 @@ def test_closure_parameter_and_data_sizes():
         c.set_data(np.zeros((8, 10)), np.zeros(8), np.zeros(8))
     with pytest.raises(ValueError):
         c.set_data(np.zeros((8, 16)), np.zeros(8), np.zeros(7))
+
+
+def test_run_history_arguments():
+    e = bare_engine()
+    with pytest.raises(ValueError):
+        e.run_history(0, 10, 1.0, every=0)
+    with pytest.raises(ValueError):
+        e.run_history(0, -1, 1.0, every=2)
