@@ -186,17 +186,13 @@ int serbf_set_gamma(serbf_engine *e, const double *gamma);
 /* Stage kernel variant: 5 (default) matrix-core tiles with the state read directly as the
  * matrix operand (FP32 and FP64, 1 or 4 tracers of 64 levels, config-mode solid-body or
  * deformational flow, hyperviscosity in every stage; other cases use 2), 2 column-vector
- * form, 3 LDS-tiled (columns of 64 levels, 1 or 4 tracers; other shapes use 2), 4
- * matrix-core tiles with the state staged in LDS (FP32, the cases of 5), 6 version 5 as a
- * persistent pipeline that overlaps each tile's RK4 update with the next tile's matrix-core
- * work (the cases of 5), 1 lane-per-level form. */
+ * form (every shape). */
 int serbf_set_kernel_version(serbf_engine *e, int version);
 /* Neighbour columns each wavefront loads before consuming them (4, 8 or 16): the
  * memory-level parallelism / register trade-off of the stage kernel. */
 int serbf_set_stage_unroll(serbf_engine *e, int unroll);
-/* Bytes of each column chunk the LDS-tiled stage kernel stages per pass (128 or 256),
- * and the tile plan in use (number of 16-node tiles, largest neighbour union). */
-int serbf_set_tile_chunk(serbf_engine *e, int bytes);
+/* The tile plan of the matrix-core stage kernel (number of 16-node tiles, largest
+ * neighbour union). */
 int serbf_tile_info(serbf_engine *e, int *ntiles, int *umax);
 /* Streams `bytes` from one device buffer to another reps times (16-byte vector
  * copy kernel); *ms_per_copy is the average launch time.  Reference for the
@@ -252,8 +248,10 @@ int serbf_closure_eval(serbf_closure *c, long first, long count, float *o);
  * lap[...] = Delta^k q (unit sphere).  Needs config-mode weights with hv_order >= 3. */
 int serbf_closure_features(serbf_engine *e, double t, double dt, float *feat, double *lap);
 /* Use the closure in the engine's time loop: the coefficients are re-evaluated from the
- * step-start state on the steps of each serbf_run call that are multiples of `every`
- * (1..4096; step 0 always), with the multiplier clamped to s_max.
+ * step-start state every `every` steps (1..4096) counted from the first step after this
+ * call, across serbf_run calls (runs of a and b steps refresh as one run of a + b steps;
+ * setting the state restarts the count, and its first step evaluates), with the
+ * multiplier clamped to s_max (0 < s_max <= 65504, the FP16 limit of the FP32 tile path).
  * c = NULL returns to the constant coefficient.  The engine keeps a pointer to c: detach
  * before destroying it. */
 int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_max);

@@ -206,6 +206,24 @@ void get_solution(void);
 void verify_solution(void);
 void init_patch_rbffd_DMs(void);
 void init_phys_constants(void);
+/* wrappers.h:10 / wrappers.c:346-395: rt_config from the ADV_* environment with the
+ * defaults of config.h:83-89, and the same parameter table on stdout (no broadcast). */
+void get_rt_config(void);
+/* wrappers.h:24 / wrappers.c:274-298: global_domains->global_nodeset from
+ * rt_config->nodeset_file, then its stencils (GPU kNN). */
+void init_global_nodeset(void);
+/* wrappers.h:18 / wrappers.c:260-268: global_layers from rt_config. */
+void init_global_layers(void);
+/* wrappers.h:26 / wrappers.c:304-322: one process, one part: every node in part 0 and
+ * the node order kept, as METIS and the contiguous reorder (domains.c:48-112) give at
+ * one part.  mpi_size other than 1 ends the process with a message. */
+void partition_global_domain(void);
+/* wrappers.h:20 / wrappers.c:222-254 with patches.c:24-346 at one part: local_patch's
+ * node set, maps, empty halos, layers and zero-filled state arrays (patch arrays with
+ * stride padded_Nvt after a CACHE_ALIGN front pad, partition arrays with padded_Nv). */
+void init_local_patch(void);
+/* wrappers.h:8 / wrappers.c:183-193: init_TC2(local_patch). */
+void get_model_ICs(void);
 
 /* Globals the reference defines in wrappers.c:23-31 and main.c:10-11. */
 extern rt_config_struct rt_config[1];

@@ -35,8 +35,19 @@ def _flags(src):
     return f
 
 
-def _compile(src):
+def _stale(src, obj):
+    """True when obj is missing or older than its source or any header it may include."""
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(d, f) for d in (CSRC, os.path.join(HERE, "..", "include"))
+                                        for f in os.listdir(d) if f.endswith(".h")]
+    return max(os.path.getmtime(d) for d in deps) > os.path.getmtime(obj)
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [hipcc()] + _flags(src) + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -44,11 +55,12 @@ def _compile(src):
     return obj
 
 
-def build(verbose=True):
+def build(verbose=True, incremental=False):
+    """Compile every source (incremental=True: only those older than their inputs) and link."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -62,7 +74,7 @@ def build(verbose=True):
 
 if __name__ == "__main__":
     try:
-        build()
+        build(incremental="--incremental" in sys.argv)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -173,6 +173,7 @@ __device__ __forceinline__ float pick4(const float o[4], int i)
 template <typename T>
 struct FeatArgs {
     StageArgs<T> s;          // state (qin), stencil, weights, basis, levels, vertical table, times
+    const int *origin;       // step index of the run's first step within the closure cadence
     const T *WL1, *WL2;
     const float *hloc;
     int ntr;
@@ -296,7 +297,7 @@ apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef, floa
     const int c = lane & 15, g = lane >> 4;
     // re-evaluate only on steps that are multiples of `every` (the step counter lives on the
     // device, so one captured graph serves every cadence); other steps exit at once
-    if ((*a.s.base + a.s.step_off) % every != 0) return;
+    if ((*a.s.base + a.s.step_off + *a.origin) % every != 0) return;
     const StageArgs<T> &s = a.s;
     unsigned char *wb = lds_raw + (size_t)w * G::wave_bytes(npad);
     unsigned short *xs = (unsigned short *)wb;
@@ -714,6 +715,7 @@ FeatArgs<T> feat_args(serbf_engine *e, StateBuffers<T> &sb, const T *q, double d
     s.tscal = tscal;
     s.base = base;
     s.step_off = step_off;
+    a.origin = e->d_base + 1;   // zero, outside the time loop
     s.N = e->N; s.n = e->n; s.Nv = e->cfg.Nv; s.nv = e->nv;
     s.flow = e->cfg.flow;
     s.ref_mode = e->cfg.mode == SERBF_MODE_REFERENCE;
@@ -808,7 +810,8 @@ int alloc_gamma(serbf_engine *e)
 template <typename T>
 int closure_enqueue(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off, bool compact)
 {
-    const FeatArgs<T> a = feat_args<T>(e, sb, sb.qT, dt, sb.tscal, e->d_base, step_off);
+    FeatArgs<T> a = feat_args<T>(e, sb, sb.qT, dt, sb.tscal, e->d_base, step_off);
+    a.origin = e->d_base + 2;
     return launch_apply<T>(e, a, sb.gamma, e->closure_every, compact ? sb.s16 : nullptr);
 }
 template int closure_enqueue<float>(serbf_engine *, StateBuffers<float> &, double, int, bool);
@@ -1019,8 +1022,9 @@ int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_
         e->hv_mode = 0;
         return SERBF_OK;
     }
-    SERBF_CHECK(every >= 1 && every <= 4096 && s_max > 0.0, SERBF_ERR_ARG,
-                "closure_attach: every 1..4096, s_max > 0");
+    // the FP32 tile kernel takes the multiplier in FP16, whose largest finite value is 65504
+    SERBF_CHECK(every >= 1 && every <= 4096 && s_max > 0.0 && s_max <= 65504.0, SERBF_ERR_ARG,
+                "closure_attach: every 1..4096, 0 < s_max <= 65504");
     SERBF_CHECK(c->have_params, SERBF_ERR_STATE, "closure_attach: closure has no parameters");
     SERBF_CHECK(c->device == e->device, SERBF_ERR_ARG, "closure_attach: closure and engine on different devices");
     SERBF_CHECK(e->have_state && e->have_lap && e->s64.WL1 && e->s32.WL1, SERBF_ERR_STATE,
@@ -1031,6 +1035,7 @@ int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_
     if (rc) return rc;
     SERBF_HIP(hipStreamSynchronize(c->stream));
     e->closure = c;
+    e->closure_start = e->steps_done;   // the first step after attaching evaluates
     e->closure_every = every;
     e->closure_smax = s_max;
     e->hv_mode = 1;

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <vector>
 
 #include "device_util.h"
@@ -97,6 +98,21 @@ struct Bound {
 {
     std::fprintf(stderr, "libserbf %s: %s\n", where, serbf_last_error());
     std::exit(1);
+}
+
+// debug.c:13-16: the reference's fatal path prints this banner and exits with status 0
+[[noreturn]] void abort_with(const char *message)
+{
+    std::printf("\n\nADVECTION SOLVER ERROR (RANK %d): %s\nABORTING SOLVER\n\n", mpi_rank, message);
+    std::fflush(stdout);
+    std::exit(0);
+}
+
+double wall_seconds()
+{
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
 serbf_engine *engine()
@@ -483,8 +499,12 @@ double q_init_TC2(double h)
 double TC2_u_r(double h, double phi)
 {
     const double w0 = TC2_params->w0, K = TC2_params->K, htop = phys_constants->htop, rho0 = phys_constants->rho0;
-    return (w0 * rho0) / (K * HS_isoT_rho(h)) * (((-2) * std::sin(K * phi) * std::sin(phi)) + (K * std::cos(K * phi) * std::cos(phi))) *
-           std::sin(SERBF_PI * h / htop);
+    // sin and cos of one argument through glibc's sincos, which a C compiler substitutes
+    // for the pair in the reference's expression; it can differ from sin/cos by 1 ulp
+    double sk, ck, sp, cp;
+    ::sincos(K * phi, &sk, &ck);
+    ::sincos(phi, &sp, &cp);
+    return (w0 * rho0) / (K * HS_isoT_rho(h)) * (((-2) * sk * sp) + (K * ck * cp)) * std::sin(SERBF_PI * h / htop);
 }
 double TC2_u_lambda(double phi) { return TC2_params->u0 * std::cos(phi); }
 double TC2_u_phi(double h, double phi)
@@ -565,6 +585,10 @@ void get_solution(void)
     if (serbf_set_state(e, q.data())) fatal("get_solution");
     if (serbf_run(e, SERBF_F64, rt_config->nsteps, rt_config->dt, 0.0)) fatal("get_solution");
     if (serbf_get_state(e, SERBF_F64, q.data())) fatal("get_solution");
+    // the reference's per-step status line (wrappers.c:56-58), once the device loop is done
+    if (mpi_rank == 0)
+        for (int tstep = 0; tstep < rt_config->nsteps; tstep++)
+            std::printf("\nSolver Status: \ttstep = %d -> t = %.1f s", tstep, tstep * rt_config->dt);
     for (int i = 0; i < N; i++) std::memcpy(p->SV_q_t->data + (size_t)i * pNvt, &q[(size_t)i * Nv], Nv * sizeof(double));
 }
 
@@ -609,6 +633,178 @@ void init_patch_rbffd_DMs(void)
     const double epsilon = c1 * std::sqrt((double)N) - c2;
     get_rbffd_DMs(local_patch, SERBF_REF_PHS_ORDER, SERBF_REF_HV_ORDER, epsilon);
 }
+
+/* wrappers.c:346-395 (one process: no broadcast).  The ADV_* environment with the
+ * defaults of config.h:83-89; prints the same parameter table. */
+void get_rt_config(void)
+{
+    const char *r;
+    r = std::getenv("ADV_NODESET_FROM_FILE");
+    rt_config->nodeset_from_file = r ? std::atoi(r) : 1;
+    r = std::getenv("ADV_NODESET_FILE");
+    std::snprintf(rt_config->nodeset_file, sizeof(rt_config->nodeset_file), "%s", r ? r : "NONE");
+    r = std::getenv("ADV_USE_METIS");
+    rt_config->use_metis = r ? std::atoi(r) : 1;
+    r = std::getenv("ADV_TEST_CASE");
+    rt_config->TC = r ? std::atoi(r) : 2;
+    r = std::getenv("ADV_STENCIL_SIZE");
+    rt_config->stencil_size = r ? std::atoi(r) : 55;
+    r = std::getenv("ADV_NUM_LAYERS");
+    rt_config->num_layers = r ? std::atoi(r) : 32;
+    r = std::getenv("ADV_MODEL_HEIGHT");
+    rt_config->model_height = r ? std::atof(r) : 12000.0;
+    r = std::getenv("ADV_TIMESTEP_LENGTH");
+    rt_config->timestep_length = r ? std::atof(r) : 600.0;
+    if (mpi_rank == 0) {
+        static const char *rule = "===============================================================================================================";
+        std::printf("\n======================================= Advection Solver Parameterizations ====================================\n\n");
+        std::printf("\tADV_NODESET_FROM_FILE: \t%d\n", rt_config->nodeset_from_file);
+        std::printf("\tADV_NODESET_FILE:      \t%s\n", rt_config->nodeset_file);
+        std::printf("\tADV_STENCIL_SIZE:      \t%d\n", rt_config->stencil_size);
+        std::printf("\tADV_USE_METIS:         \t%d\n", rt_config->use_metis);
+        std::printf("\tADV_TEST_CASE:         \t%d\n", rt_config->TC);
+        std::printf("\tADV_NUM_LAYERS:        \t%d\n", rt_config->num_layers);
+        std::printf("\tADV_MODEL_HEIGHT:      \t%.1f meters\n", rt_config->model_height);
+        std::printf("\tADV_TIMESTEP_LENGTH:   \t%.1f seconds\n", rt_config->timestep_length);
+        std::printf("\n%s\n\n", rule);
+    }
+}
+
+/* wrappers.c:274-298: the node file (read without netCDF) and its stencils (GPU kNN). */
+void init_global_nodeset(void)
+{
+    if (rt_config->nodeset_from_file != 1)
+        abort_with("Nodeset creation not yet implemented. Please provide a NetCDF file containing a valid nodeset.");
+    double t0 = wall_seconds();
+    global_domains->Nnodes = get_nodeset_from_file(rt_config->nodeset_file, global_domains->global_nodeset);
+    if (mpi_rank == 0) std::printf("\nget_nodeset_from_file() walltime -> %.2f seconds\n", wall_seconds() - t0);
+    t0 = wall_seconds();
+    get_stencils(global_domains->global_nodeset, rt_config->stencil_size);
+    if (mpi_rank == 0) std::printf("\nget_stencils() walltime -> %.2f seconds\n", wall_seconds() - t0);
+}
+
+/* wrappers.c:260-268 */
+void init_global_layers(void) { get_layers(global_layers, rt_config->model_height, rt_config->num_layers); }
+
+/* wrappers.c:304-322 at one process.  With one part, METIS puts every node in part 0
+ * and the contiguous reorder of domains.c:48-112 is the identity, so the node set
+ * keeps its order: part_ids = 0, part_sizes = {N}, part_start_ids = {0}. */
+void partition_global_domain(void)
+{
+    if (mpi_size != 1) {
+        std::fprintf(stderr, "libserbf partition_global_domain: one process per GPU engine (mpi_size %d)\n", mpi_size);
+        std::exit(1);
+    }
+    const int N = global_domains->Nnodes;
+    global_domains->Nparts = 1;
+    if (rt_config->use_metis != 1)
+        abort_with("Only METIS partitioning is currently implemented. Please set ADV_USE_METIS=1 and rerun.");
+    int *part_ids = (int *)std::calloc((size_t)N > 0 ? (size_t)N : 1, sizeof(int));
+    int *sizes = (int *)std::malloc(sizeof(int)), *starts = (int *)std::malloc(sizeof(int));
+    if (!part_ids || !sizes || !starts) fatal("partition_global_domain: malloc");
+    sizes[0] = N;
+    starts[0] = 0;
+    global_domains->part_ids = part_ids;
+    global_domains->part_sizes = sizes;
+    global_domains->part_start_ids = starts;
+}
+
+/* wrappers.c:222-254 with patches.c:24-346 at one part: the patch is the nodes that
+ * the partition's stencils touch (every node, for kNN stencils, which hold their own
+ * centre), in global order; no halos; the state arrays with the reference's strides
+ * (patch arrays padded_Nvt per column after a CACHE_ALIGN front pad, partition
+ * arrays padded_Nv), zero-filled. */
+void init_local_patch(void)
+{
+    const nodeset_struct *g = global_domains->global_nodeset;
+    const int Ng = g->Nnodes, n = g->n;
+    if (!global_domains->part_sizes || !global_domains->part_start_ids || !global_domains->part_ids)
+        abort_with("init_local_patch: partition_global_domain() has not run");
+    const int part_N = global_domains->part_sizes[0], start = global_domains->part_start_ids[0];
+    // patches.c:24-60: the nodes the partition depends on
+    std::vector<int> mask(Ng, 0);
+    for (int i = start; i < start + part_N; i++)
+        for (int j = 0; j < n; j++) mask[g->idx[(size_t)i * n + j]] = 1;
+    int Np = 0;
+    for (int gid = 0; gid < Ng; gid++) Np += mask[gid];
+    // patches.c:67-163
+    patch_struct *p = local_patch;
+    int *pid2gid = (int *)std::malloc(sizeof(int) * (Np > 0 ? Np : 1));
+    int *gid2pid = (int *)std::calloc(Ng > 0 ? Ng : 1, sizeof(int));
+    int *part_pids = (int *)std::malloc(sizeof(int) * (part_N > 0 ? part_N : 1));
+    if (!pid2gid || !gid2pid || !part_pids) fatal("init_local_patch: malloc");
+    for (int gid = 0, c = 0; gid < Ng; gid++)
+        if (mask[gid]) {
+            pid2gid[c] = gid;
+            gid2pid[gid] = c++;
+        }
+    for (int i = 0; i < part_N; i++) part_pids[i] = i;   // no left halo with one part
+    double *a[5];
+    const double *src[5] = {g->x, g->y, g->z, g->lambda, g->phi};
+    for (int k = 0; k < 5; k++) {
+        a[k] = (double *)std::malloc(sizeof(double) * (Np > 0 ? Np : 1));
+        if (!a[k]) fatal("init_local_patch: malloc");
+        for (int pid = 0; pid < Np; pid++) a[k][pid] = src[k][pid2gid[pid]];
+    }
+    double *D = (double *)std::malloc(sizeof(double) * ((size_t)Np * n > 0 ? (size_t)Np * n : 1));
+    int *idx = (int *)std::malloc(sizeof(int) * ((size_t)Np * n > 0 ? (size_t)Np * n : 1));
+    if (!D || !idx) fatal("init_local_patch: malloc");
+    for (int pid = 0; pid < Np; pid++)
+        for (int j = 0; j < n; j++) {
+            const size_t gi = (size_t)pid2gid[pid] * n + j;
+            D[(size_t)pid * n + j] = g->D[gi];
+            idx[(size_t)pid * n + j] = gid2pid[g->idx[gi]];
+        }
+    nodeset_struct *ns = p->nodeset;
+    ns->Nnodes = Np;
+    ns->n = n;
+    ns->x = a[0]; ns->y = a[1]; ns->z = a[2]; ns->lambda = a[3]; ns->phi = a[4];
+    ns->idx = idx;
+    ns->D = D;
+    p->Nnodes = Np;
+    p->part_Nnodes = part_N;
+    p->pid2gid = pid2gid;
+    p->gid2pid = gid2pid;
+    p->part_pids = part_pids;
+    // patches.c:165-265 with no neighbours: empty halo tables
+    halos_struct *h = p->halos;
+    std::memset(h, 0, sizeof(*h));
+    h->nbr_ranks = (int *)std::malloc(sizeof(int));
+    h->halo_offsets = (int *)std::calloc(1, sizeof(int));
+    h->nbr_halo_offsets = (int *)std::calloc(1, sizeof(int));
+    get_layers(p->layers, rt_config->model_height, rt_config->num_layers);
+    const int pNv = p->layers->padded_Nv, pNvt = p->layers->padded_Nvt;
+    // patches.c:300-346
+    auto patch_sv = [&](int Ndim) {
+        patch_SV_struct *sv = (patch_SV_struct *)std::calloc(1, sizeof(patch_SV_struct));
+        double *base = (double *)std::calloc((size_t)Np * pNvt * Ndim + SERBF_REF_CACHE_ALIGN, sizeof(double));
+        if (!sv || !base) fatal("init_local_patch: calloc");
+        sv->Ndim = Ndim;
+        sv->data = base + SERBF_REF_CACHE_ALIGN;
+        return sv;
+    };
+    auto part_sv = [&](int Ndim) {
+        part_SV_struct *sv = (part_SV_struct *)std::malloc(sizeof(part_SV_struct));
+        double *d = (double *)std::calloc((size_t)part_N * pNv * Ndim > 0 ? (size_t)part_N * pNv * Ndim : 1,
+                                          sizeof(double));
+        if (!sv || !d) fatal("init_local_patch: calloc");
+        sv->Ndim = Ndim;
+        sv->data = d;
+        return sv;
+    };
+    p->SV_q_init = patch_sv(1);
+    p->SV_q_t = patch_sv(1);
+    p->SV_q_k = patch_sv(1);
+    p->SV_U = part_sv(3);
+    p->SV_Up = part_sv(3);
+    p->SV_Usph = part_sv(3);
+    p->SV_F = part_sv(1);
+    p->SV_F_k = part_sv(1);
+    p->SV_3D_temp = part_sv(3);
+}
+
+/* wrappers.c:183-193: test case 2 is the only test case (config.h:12). */
+void get_model_ICs(void) { init_TC2(local_patch); }
 
 /* wrappers.c:328-340 */
 void init_phys_constants(void)

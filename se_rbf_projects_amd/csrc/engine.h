@@ -128,7 +128,11 @@ struct serbf_engine {
     double *d_wmass = nullptr;  // per-level rho(h) 4 pi r^2 dh / N
     bool f32_ready = false;   // FP32 weights/basis converted
     int tcap = 0;             // time table capacity in steps
-    int *d_base = nullptr;    // step counter read by the stage kernels
+    // [0] step counter read by the stage kernels (0 at the start of each run), [1] zero,
+    // [2] the closure cadence origin: steps run since the state was set, modulo closure_every
+    int *d_base = nullptr;
+    long long steps_done = 0; // steps run since the state was last set
+    long long closure_start = 0; // steps_done when the closure was attached
     // closure coefficient mode: 0 none/constant, 1 per-point gamma
     int hv_mode = 0;
     // hyperviscosity once per step (1) or in every stage (0)
@@ -144,8 +148,6 @@ struct serbf_engine {
     unsigned short *d_lidx = nullptr;
     int tile_ustride = 0, tile_umax = 0, ntiles = 0;
     bool have_plan = false;
-    bool plan_failed = false;
-    int tile_chunk = 128;   // bytes of each column chunk staged in LDS (128 or 256)
     // reusable scratch
     void *d_scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -172,7 +174,6 @@ int vertical_table(const serbf_config &cfg, int *start, double *W);
 int rk4_prepare(serbf_engine *e);
 // LDS-tiled stage kernel (tile.hip)
 int tile_plan(serbf_engine *e);
-bool tile_supported(const serbf_engine *e);
 // Copies between the host layout [cols][Nv] and the device layout [cols][S] (levels padded to
 // S with zeros), each as one contiguous transfer through a host staging buffer (a strided 2-D
 // copy to or from pageable memory moves one row at a time).  Both return after the transfer.

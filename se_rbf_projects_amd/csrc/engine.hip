@@ -108,8 +108,8 @@ static int engine_init(serbf_engine *e)
     SERBF_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     SERBF_HIP(hipEventCreate(&e->ev0));
     SERBF_HIP(hipEventCreate(&e->ev1));
-    SERBF_HIP(hipMalloc((void **)&e->d_base, 2 * sizeof(int)));
-    SERBF_HIP(hipMemset(e->d_base, 0, 2 * sizeof(int)));
+    SERBF_HIP(hipMalloc((void **)&e->d_base, 3 * sizeof(int)));
+    SERBF_HIP(hipMemset(e->d_base, 0, 3 * sizeof(int)));
     return SERBF_OK;
 }
 
@@ -465,6 +465,8 @@ int serbf_set_state(serbf_engine *e, const double *q)
     SERBF_HIP(hipMemcpyAsync(e->d_q0, e->s64.qT, cols * S * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
     SERBF_HIP(hipStreamSynchronize(e->stream));
     e->have_state = true;
+    e->steps_done = 0;
+    e->closure_start = 0;
     return serbf_sync_f32_from_f64(e);
 }
 
@@ -547,15 +549,8 @@ int serbf_set_hv_split(serbf_engine *e, int split)
 
 int serbf_set_kernel_version(serbf_engine *e, int version)
 {
-    SERBF_CHECK(e && version >= 1 && version <= 6, SERBF_ERR_ARG, "set_kernel_version: 1 to 6");
+    SERBF_CHECK(e && (version == 2 || version == 5), SERBF_ERR_ARG, "set_kernel_version: 2 or 5");
     e->kver = version;
-    return SERBF_OK;
-}
-
-int serbf_set_tile_chunk(serbf_engine *e, int bytes)
-{
-    SERBF_CHECK(e && (bytes == 128 || bytes == 256), SERBF_ERR_ARG, "set_tile_chunk: 128 or 256");
-    e->tile_chunk = bytes;
     return SERBF_OK;
 }
 

@@ -8,11 +8,8 @@
 // of its 16 nodes (and the Delta^k weights when the stage applies hyperviscosity) into dense
 // 16 x U blocks in LDS, multiplies them by the union's columns on the matrix cores, and applies
 // the vertical stencil, the velocity and the fused RK4 update to the result.
-//   version 4 (mtile_kernel, FP32): stages each tracer's union columns in LDS first;
-//   version 5 (mtile2_kernel, the default): reads the union columns straight from memory as
-//             the B operand, each column once per tile;
-//   version 6 (mtile3_kernel): version 5 as a persistent pipeline of matrix and update
-//             wavefronts.
+// The kernel (mtile2_kernel, stage-kernel version 5, the default) reads the union columns
+// straight from memory as the B operand, each column once per tile.
 // MFMA operand layout (16x16x4, lane l, c = l % 16, g = l / 16):
 //   A (16 x 4): A[c][g]     B (4 x 16): B[g][c]
 //   D (16 x 16): FP32 D[4g + i][c], FP64 D[g + 4i][c], i = 0..3
@@ -29,11 +26,10 @@ namespace serbf {
 namespace {
 
 constexpr int kMTileNodes = 16;
-// largest tile union (columns) the matrix-core kernels take: 320 covers the n = 84 stencils of
+// largest tile union (columns) the matrix-core kernel takes: 320 covers the n = 84 stencils of
 // uneven node sets (266 columns at most on jittered level-5 nodes); the LDS then stays within
-// 128 KB (version 4), 84 KB (version 5, FP64) and 147 KB (version 6, FP64)
+// 84 KB (FP64)
 constexpr int kMTileUcap = 320;
-constexpr int kQLD = 68;        // staged column stride (64 levels + pad)
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -48,14 +44,6 @@ __device__ __forceinline__ v4d mfma4(double a, double b, v4d c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
-// orders the LDS accesses of one wavefront
-__device__ __forceinline__ void wave_sync_m()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 template <typename T>
 struct Acc4;
 template <>
@@ -76,132 +64,6 @@ struct MTilePlan {
     const unsigned short *lidx;  // [N][n] union position of each stencil entry
     int ustride;
 };
-
-// LDS: sv[16][2] node velocities | W[16][WLD] | WL[16][WLD] (HV) | Q[UP][kQLD]
-__host__ __device__ inline int mtile_lds_floats(int up, bool hv)
-{
-    const int wld = up + 1;
-    return 32 + 16 * wld * (hv ? 2 : 1) + up * kQLD;
-}
-
-template <int STAGE, int HV>
-__global__ void __launch_bounds__(256) mtile_kernel(StageArgs<float> a, MTilePlan tp, int up)
-{
-    extern __shared__ float lds[];
-    constexpr int S = 64, NTR = 4;
-    const int wld = up + 1;
-    float *sv = lds;
-    float *W = lds + 32;
-    float *WLt = W + 16 * wld;
-    float *Q = W + 16 * wld * (HV ? 2 : 1);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int c = lane & 15, g = lane >> 4;
-    const int tile = xcd_remap(blockIdx.x, a.nblk);
-    const int p0 = tile * kMTileNodes;
-    const int U = tp.ucount[tile];
-    const int n = a.n;
-    const float *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
-
-    // ---- node velocities and the dense weight blocks ----
-    if (tid < kMTileNodes) {
-        const int p = p0 + tid < a.N ? p0 + tid : a.N - 1;
-        float ux, uy;
-        node_velocity<float>(a, a.basis + (size_t)p * kBasis, ts, ux, uy);
-        sv[2 * tid] = ux;
-        sv[2 * tid + 1] = uy;
-    }
-    for (int e = tid; e < 16 * wld * (HV ? 2 : 1); e += 256) W[e] = 0.f;
-    __syncthreads();
-    for (int e = tid; e < kMTileNodes * n; e += 256) {
-        const int t = e / n, s = e - t * n;
-        const int p = p0 + t;
-        if (p < a.N) {
-            const size_t o = (size_t)p * n + s;
-            const int k = tp.lidx[o];
-            W[t * wld + k] = sv[2 * t] * a.Wx[o] + sv[2 * t + 1] * a.Wy[o];
-            if (HV) WLt[t * wld + k] = a.WL[o];
-        }
-    }
-    const int *uni = tp.uni + (size_t)tile * tp.ustride;
-
-    for (int tr = 0; tr < NTR; tr++) {
-        // ---- stage the union's columns of tracer tr (rows U..up-1 are zero) ----
-        __syncthreads();   // weights written / previous tracer's reads done
-        for (int e = tid; e < up * 16; e += 256) {
-            const int k = e >> 4, j4 = (e & 15) * 4;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (k < U) v = *(const float4 *)(a.qin + ((size_t)uni[k] * NTR + tr) * S + j4);
-            float *d = Q + k * kQLD + j4;
-            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
-        }
-        __syncthreads();
-
-        // ---- W (16 x up) x Q (up x 16 levels of wavefront w) ----
-        v4f acc = {0.f, 0.f, 0.f, 0.f}, accl = {0.f, 0.f, 0.f, 0.f};
-        const float *wa = W + c * wld + g;
-        const float *wl = WLt + c * wld + g;
-        const float *qb = Q + g * kQLD + 16 * w + c;
-        for (int k0 = 0; k0 < up; k0 += 4) {
-            const float b = qb[k0 * kQLD];
-            acc = mfma4(wa[k0], b, acc);
-            if (HV) accl = mfma4(wl[k0], b, accl);
-        }
-
-        // ---- vertical stencil, velocity, RK4 update: 4 nodes x 1 level per lane ----
-        const int v = 16 * w + c;
-        const float r_inv = a.lev[v];
-        const int vst = a.vstart[v < a.Nv ? v : 0];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int t = 4 * g + i;
-            const int p = p0 + t;
-            if (p >= a.N) continue;
-            // the node's own column: its staged row when stencil entry 0 is the node itself
-            // (kNN stencils), else global memory
-            const int ks = tp.lidx[(size_t)p * n];
-            const float *own = uni[ks] == p ? Q + ks * kQLD : a.qin + ((size_t)p * NTR + tr) * S;
-            float z = 0.f;
-            if (v < a.Nv)
-                for (int j = 0; j < a.nv; j++) {
-                    const int lv = vst + j;
-                    if (lv >= 0 && lv < a.Nv) z = fmaf(a.vW[v * kMaxNv + j], own[lv], z);
-                }
-            float upx, upy, upz;
-            level_velocity<float, S>(a, a.basis + (size_t)p * kBasis, ts, p, v, upx, upy, upz);
-            (void)upx;
-            (void)upy;
-            const size_t my = ((size_t)p * NTR + tr) * S + v;
-            float f = a.alpha * (acc[i] * r_inv + upz * z);
-            if (HV) f += (a.hv_mode ? a.gamma[my] : a.hv_coef) * accl[i];
-            const bool live = v < a.Nv;
-            if (!live) f = 0.f;
-            if (STAGE == 0) {
-                a.F[my] = f;
-            } else if (STAGE == 1) {
-                a.F[my] = f;
-                a.qout[my] = live ? own[v] + a.hdt * f : 0.f;
-            } else if (STAGE == 2 || STAGE == 3) {
-                const float h = STAGE == 2 ? a.hdt : a.dt;
-                a.F[my] = a.F[my] + 2.f * f;
-                a.qout[my] = live ? a.qT[my] + h * f : 0.f;
-            } else {
-                a.qout[my] = live ? a.qT[my] + a.dt6 * (a.F[my] + 1.f * f) : 0.f;
-            }
-        }
-    }
-}
-
-template <int STAGE, int HV>
-void launch_one(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp, int up, int bytes)
-{
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)mtile_kernel<STAGE, HV>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        attr = true;
-    }
-    hipLaunchKernelGGL((mtile_kernel<STAGE, HV>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
-}
 
 // ---------------------------------------------------------------------------
 // Version 5 (FP32 on v_mfma_f32_16x16x4f32, FP64 on v_mfma_f64_16x16x4f64): the same
@@ -450,273 +312,6 @@ mtile2_kernel(StageArgs<T> a, MTilePlan tp, int up)
     }
 }
 
-// ---------------------------------------------------------------------------
-// Version 6: version 5 as a persistent, wavefront-specialised pipeline.
-//
-// A work-group of 12 wavefronts walks a strided sequence of tiles of its XCD's contiguous
-// range.  Wavefronts 0-7 (the matrix group; wavefront w takes level block w % 4 of tracers
-// 2 (w / 4) and 2 (w / 4) + 1) build the dense weight blocks of tile i and run the matrix-core
-// loop of version 5 on it, with the union-column loads of the next step in flight during the
-// MFMAs of the current one; wavefronts 8-11 (the update group) run the vertical stencil,
-// velocity and RK4 update of tile i-1 meanwhile.  The matrix group hands over, per point, the
-// horizontal part of the right-hand side, alpha r_inv (W q) + c (W_L q), through a double-buffered
-// LDS array.  Each tile takes two phases, each closed by a work-group barrier:
-//   phase 1: matrix group: clear and fill the weight blocks and union offsets of tile i (each
-//            matrix wavefront owns 2 rows of the blocks, so its clearing and filling are
-//            ordered within the wavefront) | update group: nodes 0-1 of each wavefront of tile i-1
-//   phase 2: matrix group: matrix-core loop of tile i, hand-over written to buffer i % 2 |
-//            update group: nodes 2-3 of each wavefront of tile i-1 (read from buffer (i-1) % 2)
-// So the HBM-bound update streams of one tile overlap the L2 gathers and matrix-core work of
-// the next, which a one-tile-per-work-group kernel leaves to chance.  The update group holds a
-// column per lane (lane = level) and takes the vertical stencil's neighbours by cross-lane
-// shuffles, so nothing of the state is staged in LDS.
-__host__ __device__ inline int mtile3_lds_bytes(int up, bool hv, int tsize)
-{
-    const int wld = up + 1;
-    const int wreg = 16 * wld * (hv ? 2 : 1);
-    const int dreg = 2 * kMTileNodes * 4 * 64;
-    return (wreg + dreg) * tsize + up * (int)sizeof(int);
-}
-
-template <typename T, int STAGE, int HV>
-__device__ __forceinline__ void m3_update_node(const StageArgs<T> &a, const T *Dbuf, const T *ts, int p0, int nd,
-                                               int v)
-{
-    constexpr int S = 64, NTR = 4;
-    const int p = p0 + nd;
-    if (p >= a.N) return;   // wavefront-uniform
-    const bool live = v < a.Nv;
-    const int vst = a.vstart[live ? v : 0];
-    T upx, upy, upz;
-    level_velocity<T, S>(a, a.basis + (size_t)p * kBasis, ts, p, live ? v : 0, upx, upy, upz);
-    (void)upx;
-    (void)upy;
-    // all loads before the first store (see the version-5 update)
-    T selfv[NTR], Fv[NTR], Qv[NTR];
-#pragma unroll
-    for (int t = 0; t < NTR; t++) {
-        const size_t my = ((size_t)p * NTR + t) * S + v;
-        selfv[t] = a.qin[my];
-        Fv[t] = STAGE >= 2 ? a.F[my] : T(0);
-        Qv[t] = STAGE >= 2 ? a.qT[my] : T(0);
-    }
-#pragma unroll
-    for (int t = 0; t < NTR; t++) {
-        const size_t my = ((size_t)p * NTR + t) * S + v;
-        const T self = selfv[t];
-        T z = T(0);
-#pragma unroll
-        for (int jj = 0; jj < kMaxNv; jj++) {
-            const T qj = __shfl(self, (vst + jj) & 63, 64);
-            if (live && jj < a.nv) z = fma(a.vW[v * kMaxNv + jj], qj, z);
-        }
-        T f = Dbuf[(nd * NTR + t) * S + v] + a.alpha * upz * z;
-        if (!live) f = T(0);
-        if (STAGE == 0) {
-            a.F[my] = f;
-        } else if (STAGE == 1) {
-            a.F[my] = f;
-            a.qout[my] = live ? self + a.hdt * f : T(0);
-        } else if (STAGE == 2 || STAGE == 3) {
-            const T h = STAGE == 2 ? a.hdt : a.dt;
-            a.F[my] = Fv[t] + T(2) * f;
-            a.qout[my] = live ? Qv[t] + h * f : T(0);
-        } else {
-            a.qout[my] = live ? Qv[t] + a.dt6 * (Fv[t] + T(1) * f) : T(0);
-        }
-    }
-}
-
-template <typename T, int STAGE, int HV>
-__global__ void __launch_bounds__(768) mtile3_kernel(StageArgs<T> a, MTilePlan tp, int up, int ntiles)
-{
-    extern __shared__ __align__(16) unsigned char m3_lds[];
-    typedef typename Acc4<T>::type VT;
-    constexpr int S = 64, NTR = 4, MW = 8, TPW = 2, MTH = 64 * MW;
-    const int wld = up + 1;
-    const int wreg = 16 * wld * (HV ? 2 : 1);
-    T *W = (T *)m3_lds;
-    T *WLt = W + 16 * wld;
-    T *D = W + wreg;                                   // [2][16 nodes][4 tracers][64 levels]
-    int *uoff = (int *)(D + 2 * kMTileNodes * NTR * S);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const bool mgroup = wv < MW;
-    const int n = a.n;
-    const T *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
-
-    // this work-group's tiles: XCD x = blockIdx % 8 owns a contiguous range; its G work-groups
-    // take every G-th tile of it
-    const int x = blockIdx.x & 7, j = blockIdx.x >> 3, G = gridDim.x >> 3;
-    const int q = ntiles >> 3, r = ntiles & 7;
-    const int t_begin = x * q + (x < r ? x : r), t_end = t_begin + q + (x < r ? 1 : 0);
-    const int nmine = t_begin + j < t_end ? (t_end - 1 - (t_begin + j)) / G + 1 : 0;
-
-    for (int it = 0; it <= nmine; it++) {
-        const int tc = t_begin + j + it * G;           // matrix tile (it < nmine)
-        const int tu = tc - G;                          // update tile (it > 0)
-        // ---- phase 1 ----
-        if (mgroup) {
-            if (it < nmine) {
-                const int p0 = tc * kMTileNodes;
-                const int U = tp.ucount[tc];
-                const int u16 = (U + 15) & ~15;
-                const int *uni = tp.uni + (size_t)tc * tp.ustride;
-                for (int k = tid; k < u16; k += MTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
-                // rows 2 wv, 2 wv + 1: clear, then scatter this tile's stencils
-                for (int e = lane; e < 2 * wld; e += 64) {
-                    W[2 * wv * wld + e] = T(0);
-                    if (HV) WLt[2 * wv * wld + e] = T(0);
-                }
-                wave_sync_m();
-                for (int e = lane; e < 2 * n; e += 64) {
-                    const int t = 2 * wv + e / n, s = e % n;
-                    const int p = p0 + t;
-                    if (p < a.N) {
-                        const size_t o = (size_t)p * n + s;
-                        T ux, uy;
-                        node_velocity<T>(a, a.basis + (size_t)p * kBasis, ts, ux, uy);
-                        const int pos = t * wld + tp.lidx[o];
-                        W[pos] = ux * a.Wx[o] + uy * a.Wy[o];
-                        if (HV) WLt[pos] = a.WL[o];
-                    }
-                }
-            }
-        } else if (it > 0) {
-            const T *Db = D + (size_t)((it - 1) & 1) * kMTileNodes * NTR * S;
-            for (int k = 0; k < 2; k++) m3_update_node<T, STAGE, HV>(a, Db, ts, tu * kMTileNodes, 4 * (wv - MW) + k, lane);
-        }
-        __syncthreads();
-        // ---- phase 2 ----
-        if (mgroup) {
-            if (it < nmine) {
-                const int c = lane & 15, g = lane >> 4, w = wv & 3, t0 = (wv >> 2) * TPW;
-                const int p0 = tc * kMTileNodes;
-                const int u16 = (tp.ucount[tc] + 15) & ~15;
-                const int vl = 16 * w + c;
-                VT acc[TPW], accl[TPW];
-#pragma unroll
-                for (int t = 0; t < TPW; t++) {
-                    acc[t] = VT{T(0), T(0), T(0), T(0)};
-                    accl[t] = VT{T(0), T(0), T(0), T(0)};
-                }
-                const T *qw = a.qin + t0 * S + vl;
-                const T *wa = W + c * wld + g;
-                const T *wl = WLt + c * wld + g;
-                T b[4][TPW];
-#pragma unroll
-                for (int jj = 0; jj < 4; jj++) {
-                    const int o = uoff[4 * jj + g];
-#pragma unroll
-                    for (int t = 0; t < TPW; t++) b[jj][t] = qw[o + t * S];
-                }
-                for (int k0 = 0; k0 < u16; k0 += 16) {
-                    // the next step's loads are issued before this step's MFMAs
-                    T bn[4][TPW], av[4], al[4];
-                    const bool more = k0 + 16 < u16;   // wavefront-uniform
-                    if (more) {
-#pragma unroll
-                        for (int jj = 0; jj < 4; jj++) {
-                            const int o = uoff[k0 + 16 + 4 * jj + g];
-#pragma unroll
-                            for (int t = 0; t < TPW; t++) bn[jj][t] = qw[o + t * S];
-                        }
-                    }
-#pragma unroll
-                    for (int jj = 0; jj < 4; jj++) {
-                        av[jj] = wa[k0 + 4 * jj];
-                        if (HV) al[jj] = wl[k0 + 4 * jj];
-                    }
-#pragma unroll
-                    for (int jj = 0; jj < 4; jj++)
-#pragma unroll
-                        for (int t = 0; t < TPW; t++) {
-                            acc[t] = mfma4(av[jj], b[jj][t], acc[t]);
-                            if (HV) accl[t] = mfma4(al[jj], b[jj][t], accl[t]);
-                        }
-                    if (more) {
-#pragma unroll
-                        for (int jj = 0; jj < 4; jj++)
-#pragma unroll
-                            for (int t = 0; t < TPW; t++) b[jj][t] = bn[jj][t];
-                    }
-                }
-                T *Db = D + (size_t)(it & 1) * kMTileNodes * NTR * S;
-                const T ar = a.alpha * a.lev[vl];
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int nd = Acc4<T>::row(g, i);
-                    const int p = p0 + nd < a.N ? p0 + nd : a.N - 1;
-#pragma unroll
-                    for (int t = 0; t < TPW; t++) {
-                        T h = ar * acc[t][i];
-                        if (HV)
-                            h += (a.hv_mode ? a.gamma[((size_t)p * NTR + t0 + t) * S + vl] : a.hv_coef) * accl[t][i];
-                        Db[(nd * NTR + t0 + t) * S + vl] = h;
-                    }
-                }
-            }
-        } else if (it > 0) {
-            const T *Db = D + (size_t)((it - 1) & 1) * kMTileNodes * NTR * S;
-            for (int k = 2; k < 4; k++) m3_update_node<T, STAGE, HV>(a, Db, ts, tu * kMTileNodes, 4 * (wv - MW) + k, lane);
-        }
-        __syncthreads();
-    }
-}
-
-template <typename T, int STAGE, int HV>
-void launch_three(serbf_engine *e, const StageArgs<T> &a, const MTilePlan &tp, int up, int bytes, int grid)
-{
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void *)mtile3_kernel<T, STAGE, HV>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
-    }
-    hipLaunchKernelGGL((mtile3_kernel<T, STAGE, HV>), dim3(grid), dim3(768), bytes, e->stream, a, tp, up, e->ntiles);
-}
-
-template <typename T>
-int launch_mtile3(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTilePlan &tp)
-{
-    const int up = (e->tile_umax + 15) & ~15;
-    const int bytes = mtile3_lds_bytes(up, hv != 0, (int)sizeof(T));
-    SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "matrix-core stage kernel: tile union too large");
-    SERBF_CHECK(e->n <= 128, SERBF_ERR_UNSUPPORTED, "matrix-core stage kernel: n > 128");
-    if (e->cus <= 0) {
-        hipDeviceProp_t prop;
-        e->cus = hipGetDeviceProperties(&prop, e->device) == hipSuccess ? prop.multiProcessorCount : 256;
-    }
-    // resident work-groups only (the kernel is persistent), a multiple of 8; the occupancy of
-    // the stage-1 instantiation (with its LDS limit raised first) stands for all stages, and is
-    // computed once per LDS size
-    static int per_cu = 0, per_cu_bytes = -1;
-    if (bytes != per_cu_bytes) {
-        (void)hipFuncSetAttribute((const void *)mtile3_kernel<T, 1, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        int pc = 0;
-        SERBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, (const void *)mtile3_kernel<T, 1, 1>, 768, bytes));
-        per_cu = pc < 1 ? 1 : pc;
-        per_cu_bytes = bytes;
-    }
-    int grid = e->cus * per_cu;
-    grid = grid / 8 * 8;
-    if (grid < 8) grid = 8;
-    switch (stage * 2 + (hv ? 1 : 0)) {
-    case 0: launch_three<T, 0, 0>(e, a, tp, up, bytes, grid); break;
-    case 1: launch_three<T, 0, 1>(e, a, tp, up, bytes, grid); break;
-    case 2: launch_three<T, 1, 0>(e, a, tp, up, bytes, grid); break;
-    case 3: launch_three<T, 1, 1>(e, a, tp, up, bytes, grid); break;
-    case 4: launch_three<T, 2, 0>(e, a, tp, up, bytes, grid); break;
-    case 5: launch_three<T, 2, 1>(e, a, tp, up, bytes, grid); break;
-    case 6: launch_three<T, 3, 0>(e, a, tp, up, bytes, grid); break;
-    case 7: launch_three<T, 3, 1>(e, a, tp, up, bytes, grid); break;
-    case 8: launch_three<T, 4, 0>(e, a, tp, up, bytes, grid); break;
-    default: launch_three<T, 4, 1>(e, a, tp, up, bytes, grid); break;
-    }
-    SERBF_HIP(hipGetLastError());
-    return SERBF_OK;
-}
-
 // tracers per wavefront of the version-5 kernel: 2 (8 wavefronts per work-group).  Measured at
 // 655362 x 60 x 4 with the batched update loads, before the occupancy caps: FP32 1.44 ms per
 // stage (4 tracers per wavefront: 1.44 ms), FP64 3.08 ms (4 tracers: 3.57 ms, one wavefront
@@ -770,10 +365,10 @@ int launch_mtile2(serbf_engine *e, int stage, int hv, StageArgs<T> a, const MTil
 
 // The matrix-core stage kernels serve 4 tracers (version 5 also 1 tracer) of 64 levels, flows
 // whose horizontal velocity does not depend on the level (config mode), and tile plans whose
-// unions fit (version 4: FP32; versions 5 and 6: FP32 and FP64).
+// unions fit (FP32 and FP64).
 bool mtile_supported(const serbf_engine *e)
 {
-    return (e->ntr == 4 || (e->ntr == 1 && e->kver == 5)) && e->S == 64 && e->cfg.mode == SERBF_MODE_CONFIG &&
+    return (e->ntr == 4 || e->ntr == 1) && e->S == 64 && e->cfg.mode == SERBF_MODE_CONFIG &&
            (e->cfg.flow == SERBF_FLOW_DEFORM || e->cfg.flow == SERBF_FLOW_SOLID_BODY) && e->have_plan &&
            e->tile_umax <= kMTileUcap && (size_t)e->N * e->ntr * e->S < (size_t)INT_MAX;
 }
@@ -789,28 +384,7 @@ int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<T> a)
     tp.lidx = e->d_lidx;
     tp.ustride = e->tile_ustride;
     a.nblk = e->ntiles;
-    if (e->kver == 6) return launch_mtile3<T>(e, stage, hv, a, tp);
-    if constexpr (std::is_same<T, double>::value) {
-        return launch_mtile2<double>(e, stage, hv, a, tp);
-    } else {
-        if (e->kver == 5) return launch_mtile2<float>(e, stage, hv, a, tp);
-        const int up = e->tile_umax;   // multiple of 4
-        const int bytes = mtile_lds_floats(up, hv != 0) * (int)sizeof(float);
-        switch (stage * 2 + (hv ? 1 : 0)) {
-        case 0: launch_one<0, 0>(e, a, tp, up, bytes); break;
-        case 1: launch_one<0, 1>(e, a, tp, up, bytes); break;
-        case 2: launch_one<1, 0>(e, a, tp, up, bytes); break;
-        case 3: launch_one<1, 1>(e, a, tp, up, bytes); break;
-        case 4: launch_one<2, 0>(e, a, tp, up, bytes); break;
-        case 5: launch_one<2, 1>(e, a, tp, up, bytes); break;
-        case 6: launch_one<3, 0>(e, a, tp, up, bytes); break;
-        case 7: launch_one<3, 1>(e, a, tp, up, bytes); break;
-        case 8: launch_one<4, 0>(e, a, tp, up, bytes); break;
-        default: launch_one<4, 1>(e, a, tp, up, bytes); break;
-        }
-        SERBF_HIP(hipGetLastError());
-        return SERBF_OK;
-    }
+    return launch_mtile2<T>(e, stage, hv, a, tp);
 }
 template int launch_mtile<float>(serbf_engine *, int, int, StageArgs<float>);
 template int launch_mtile<double>(serbf_engine *, int, int, StageArgs<double>);

@@ -12,15 +12,12 @@
 //     stage 3: F  = F + 2 f;    q_B = q_T + dt f
 //     stage 4: q_T = q_T + dt/6 (F + 1.0 f)
 //
-// Mapping: one wavefront per node column (LANES = 64) or 64/LANES columns per
-// wavefront for short columns; lane = vertical level.  The stencil indices and
-// weights of a node are wave-uniform (scalar loads); each neighbour's column is
-// one coalesced vector load of LANES contiguous values ([node][tracer][level]
-// layout, column stride LANES).  The vertical stencil is applied to the node's
-// own column (stencil entry 0) with cross-lane permutes.  The velocity is
-// evaluated in-kernel from a per-node basis and per-level tables (no velocity
-// arrays are streamed).  Work-groups are remapped so that each XCD processes a
-// contiguous range of nodes (its L2 sees the spatially local neighbour columns).
+// Two forms of the stage: the matrix-core tiles of mtile.hip (version 5, the default where
+// its shape applies: config-mode flows, 1 or 4 tracers of 64 levels) and the column-vector
+// kernel below (version 2), which serves every shape.  The velocity is evaluated in-kernel
+// from a per-node basis and per-level tables (no velocity arrays are streamed).
+// Work-groups are remapped so that each XCD processes a contiguous range of nodes (its L2
+// sees the spatially local neighbour columns).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -34,143 +31,6 @@
 #include "stage_common.h"
 
 namespace serbf {
-
-template <typename T, int LANES, int STAGE, int NTR, bool HV>
-__global__ void __launch_bounds__(256) stage_kernel(StageArgs<T> a)
-{
-    constexpr int NPW = 64 / LANES;
-    const int lane = threadIdx.x & 63;
-    const int blk = xcd_remap(blockIdx.x, a.nblk);
-    const int wave = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int sub = lane / LANES;
-    const int v = lane % LANES;
-    int p = wave * NPW + sub;
-    if (NPW == 1) {
-        p = __builtin_amdgcn_readfirstlane(p);
-        if (p >= a.N)
-            return;
-    }
-    const bool node_ok = p < a.N;
-    const int pc = node_ok ? p : a.N - 1;
-    const bool lvl_ok = v < a.Nv;
-    const int n = a.n;
-
-    const int *__restrict__ ip = a.idx + (size_t)pc * n;
-    const T *__restrict__ wxp = a.Wx + (size_t)pc * n;
-    const T *__restrict__ wyp = a.Wy + (size_t)pc * n;
-    const T *__restrict__ wlp = a.WL + (size_t)pc * n;
-
-    T ax[NTR], ay[NTR], al[NTR], self[NTR];
-#pragma unroll
-    for (int t = 0; t < NTR; t++) {
-        ax[t] = T(0); ay[t] = T(0); al[t] = T(0); self[t] = T(0);
-    }
-
-    // neighbour 0 is the node itself (self-first stencils)
-    {
-        const int nb = ip[0];
-        const T wx = wxp[0], wy = wyp[0];
-        const T *col = a.qin + (size_t)nb * NTR * LANES + v;
-#pragma unroll
-        for (int t = 0; t < NTR; t++) {
-            const T b = col[t * LANES];
-            self[t] = b;
-            ax[t] = wx * b;
-            ay[t] = wy * b;
-            if (HV) al[t] = wlp[0] * b;
-        }
-    }
-#pragma unroll 4
-    for (int s = 1; s < n; s++) {
-        const int nb = ip[s];
-        const T wx = wxp[s], wy = wyp[s];
-        T wl = T(0);
-        if (HV) wl = wlp[s];
-        const T *col = a.qin + (size_t)nb * NTR * LANES + v;
-#pragma unroll
-        for (int t = 0; t < NTR; t++) {
-            const T b = col[t * LANES];
-            ax[t] = fma(wx, b, ax[t]);
-            ay[t] = fma(wy, b, ay[t]);
-            if (HV) al[t] = fma(wl, b, al[t]);
-        }
-    }
-
-    // vertical stencil on the own column via cross-lane permutes
-    T az[NTR];
-#pragma unroll
-    for (int t = 0; t < NTR; t++) az[t] = T(0);
-    {
-        const int vs = lvl_ok ? a.vstart[v] : 0;
-        const T *wv = a.vW + v * kMaxNv;
-        for (int j = 0; j < a.nv; j++) {
-            const int src = vs + j;
-            const bool ok = lvl_ok && src >= 0 && src < a.Nv;
-            const int sl = sub * LANES + (ok ? src : 0);
-            const T w = ok ? wv[j] : T(0);
-#pragma unroll
-            for (int t = 0; t < NTR; t++) {
-                const T b = __shfl(self[t], sl, 64);
-                az[t] = fma(w, b, az[t]);
-            }
-        }
-    }
-
-    // velocity in the node's rotated frame at this level
-    const T *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
-    const T *B = a.basis + (size_t)pc * kBasis;
-    const T r_inv = a.lev[v];
-    T upx, upy, upz;
-    if (a.flow == VEL_TC2) {
-        const T wt = ts[0];
-        const T ga = wt * a.lev[LANES + v], gb = wt * a.lev[2 * LANES + v];
-        upx = B[0] + (ga * B[3] + gb * B[6]);
-        upy = B[1] + (ga * B[4] + gb * B[7]);
-        upz = a.ref_mode ? B[2] + (ga * B[5] + gb * B[8]) : gb * B[9];
-    } else if (a.flow == VEL_DEFORM) {
-        const T c = ts[0], C2 = ts[1], S2 = ts[2];
-        const T cl = T(0.5) - T(0.5) * (B[9] * C2 + B[10] * S2);
-        const T cp = B[10] * C2 - B[9] * S2;
-        const T w = c * a.lev[LANES + v];
-        upx = B[0] + c * (cl * B[3] + cp * B[6]) + w * B[11];
-        upy = B[1] + c * (cl * B[4] + cp * B[7]) + w * B[12];
-        upz = a.ref_mode ? B[2] + c * (cl * B[5] + cp * B[8]) + w * B[13] : w;
-    } else if (a.flow == VEL_SOLID) {
-        upx = B[0];
-        upy = B[1];
-        upz = a.ref_mode ? B[2] : T(0);
-    } else {
-        const T *u = a.uarr + (size_t)pc * 3 * LANES + v;
-        upx = u[0];
-        upy = u[LANES];
-        upz = u[2 * LANES];
-    }
-
-    if (!(node_ok && lvl_ok))
-        return;
-    const size_t base = (size_t)p * NTR * LANES + v;
-#pragma unroll
-    for (int t = 0; t < NTR; t++) {
-        const T gx = ax[t] * r_inv, gy = ay[t] * r_inv;
-        T f = a.alpha * ((upx * gx + upy * gy) + upz * az[t]);
-        if (HV) f += (a.hv_mode ? a.gamma[base + t * LANES] : a.hv_coef) * al[t];
-        const size_t o = base + t * LANES;
-        if (STAGE == 0) {
-            a.F[o] = f;
-        } else if (STAGE == 1) {
-            a.F[o] = f;
-            a.qout[o] = self[t] + a.hdt * f;
-        } else if (STAGE == 2) {
-            a.F[o] = a.F[o] + T(2) * f;
-            a.qout[o] = a.qT[o] + a.hdt * f;
-        } else if (STAGE == 3) {
-            a.F[o] = a.F[o] + T(2) * f;
-            a.qout[o] = a.qT[o] + a.dt * f;
-        } else {
-            a.qout[o] = a.qT[o] + a.dt6 * (a.F[o] + T(1) * f);
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Stage kernel, column-vector form.
@@ -626,35 +486,6 @@ int ensure_tscal(serbf_engine *e, StateBuffers<T> &sb, int nsteps, double dt, do
     return upload_converted<T>(e, sb.tscal, tab.data(), tab.size());
 }
 
-template <typename T, int LANES, int NTR, bool HV>
-void launch_one(int stage, dim3 grid, const StageArgs<T> &a, hipStream_t s)
-{
-    switch (stage) {
-    case 0: hipLaunchKernelGGL((stage_kernel<T, LANES, 0, NTR, HV>), grid, dim3(256), 0, s, a); break;
-    case 1: hipLaunchKernelGGL((stage_kernel<T, LANES, 1, NTR, HV>), grid, dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel<T, LANES, 2, NTR, HV>), grid, dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((stage_kernel<T, LANES, 3, NTR, HV>), grid, dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((stage_kernel<T, LANES, 4, NTR, HV>), grid, dim3(256), 0, s, a); break;
-    }
-}
-
-template <typename T, int LANES>
-void launch_lanes(serbf_engine *e, int stage, bool hv, StageArgs<T> a)
-{
-    const int npw = 64 / LANES;
-    const int nwaves = (e->N + npw - 1) / npw;
-    const int nblk = (nwaves + 3) / 4;
-    a.nblk = nblk;
-    const dim3 grid(nblk);
-    if (e->ntr == 1) {
-        if (hv) launch_one<T, LANES, 1, true>(stage, grid, a, e->stream);
-        else launch_one<T, LANES, 1, false>(stage, grid, a, e->stream);
-    } else {
-        if (hv) launch_one<T, LANES, 4, true>(stage, grid, a, e->stream);
-        else launch_one<T, LANES, 4, false>(stage, grid, a, e->stream);
-    }
-}
-
 template <typename T, int S, int NTR, int U>
 void launch2u(serbf_engine *e, int stage, int hv, bool xy, StageArgs<T> a)
 {
@@ -700,31 +531,18 @@ void launch2(serbf_engine *e, int stage, int hv, bool xy, StageArgs<T> a)
 }
 
 // hv: 0 none, 1 hyperviscosity in this stage, 2 split (stage 1 only)
+// Version 5 (matrix-core tiles) where its shape applies, else the column kernel (version 2).
 template <typename T>
 int launch_stage(serbf_engine *e, int stage, int hv, const StageArgs<T> &a)
 {
-    if constexpr (std::is_same<T, float>::value) {
-        if (e->kver == 4 && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
-    }
-    if ((e->kver == 5 || e->kver == 6) && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
-    if (e->kver == 3 && e->have_plan && !e->plan_failed && tile_supported(e)) {
-        const bool xy = a.flow == VEL_TC2 || a.flow == VEL_ARRAY || (a.ref_mode && a.flow == VEL_DEFORM);
-        launch_tile<T>(e, stage, hv, xy, a);
-    } else if (e->kver == 1) {
-        switch (e->S) {
-        case 16: launch_lanes<T, 16>(e, stage, hv != 0, a); break;
-        case 32: launch_lanes<T, 32>(e, stage, hv != 0, a); break;
-        default: launch_lanes<T, 64>(e, stage, hv != 0, a); break;
-        }
+    if (e->kver == 5 && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
+    const bool xy = a.flow == VEL_TC2 || a.flow == VEL_ARRAY || (a.ref_mode && a.flow == VEL_DEFORM);
+    if (e->S == 16) {
+        if (e->ntr == 1) launch2<T, 16, 1>(e, stage, hv, xy, a);
+        else launch2<T, 16, 4>(e, stage, hv, xy, a);
     } else {
-        const bool xy = a.flow == VEL_TC2 || a.flow == VEL_ARRAY || (a.ref_mode && a.flow == VEL_DEFORM);
-        if (e->S == 16) {
-            if (e->ntr == 1) launch2<T, 16, 1>(e, stage, hv, xy, a);
-            else launch2<T, 16, 4>(e, stage, hv, xy, a);
-        } else {
-            if (e->ntr == 1) launch2<T, 64, 1>(e, stage, hv, xy, a);
-            else launch2<T, 64, 4>(e, stage, hv, xy, a);
-        }
+        if (e->ntr == 1) launch2<T, 64, 1>(e, stage, hv, xy, a);
+        else launch2<T, 64, 4>(e, stage, hv, xy, a);
     }
     SERBF_HIP(hipGetLastError());
     return SERBF_OK;
@@ -797,10 +615,12 @@ int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, doubl
 {
     int rc = ensure_tscal<T>(e, sb, nsteps, dt, t0);
     if (rc) return rc;
-    if (e->kver == 3 && tile_supported(e) && (rc = tile_plan(e))) return rc;
-    if (e->kver >= 4 && (e->ntr == 4 || e->ntr == 1) && e->S == 64 && e->n <= 128 && (rc = tile_plan(e))) return rc;
-    const int zero = 0;
-    SERBF_HIP(hipMemcpyAsync(e->d_base, &zero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    if (e->kver == 5 && (e->ntr == 4 || e->ntr == 1) && e->S == 64 && e->n <= 128 && (rc = tile_plan(e))) return rc;
+    // step counter 0 and the closure cadence continued from the steps already run on this
+    // state, so that runs of a and b steps refresh the closure as one run of a + b steps
+    const int every = e->closure ? e->closure_every : 1;
+    const int head[3] = {0, 0, (int)((e->steps_done - e->closure_start) % every)};
+    SERBF_HIP(hipMemcpyAsync(e->d_base, head, sizeof(head), hipMemcpyHostToDevice, e->stream));
     // the graph of kGraphSteps steps reads the step counter and bumps it at the end
     const void *key = (const void *)sb.tscal;
     const int hvk = (hv_enabled(e) ? 1 : 0) + 2 * e->hv_mode + 4 * e->hv_split + 8 * e->kver + 32 * e->unroll +
@@ -831,6 +651,7 @@ int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, doubl
         if ((rc = enqueue_step<T>(e, sb, dt, j))) return rc;
     if (timed) SERBF_HIP(hipEventRecord(e->ev1, e->stream));
     SERBF_HIP(hipStreamSynchronize(e->stream));
+    e->steps_done += nsteps;
     if (timed) {
         float t = 0.f;
         SERBF_HIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
@@ -893,10 +714,6 @@ int rk4_rhs(serbf_engine *e, double t, double *F_host)
     DevTemp<double> d_ts;
     SERBF_HIP(d_ts.alloc(4 * kTimeScalars));
     SERBF_HIP(hipMemcpyAsync(d_ts.p, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
-    if (e->kver == 3 && tile_supported(e)) {
-        const int prc = tile_plan(e);
-        if (prc) return prc;
-    }
     StageArgs<double> a = base_args<double>(e, e->s64, 0.0);
     a.tscal = d_ts.p;
     a.base = e->d_base + 1;   // always zero
@@ -926,10 +743,6 @@ int rk4_eval_array_velocity(serbf_engine *e, const double *Up_host, double alpha
     DevTemp<double> d_ts;
     SERBF_HIP(d_ts.alloc(4 * kTimeScalars));
     SERBF_HIP(hipMemcpyAsync(d_ts.p, ts, sizeof(ts), hipMemcpyHostToDevice, e->stream));
-    if (e->kver == 3 && tile_supported(e)) {
-        const int prc = tile_plan(e);
-        if (prc) return prc;
-    }
     StageArgs<double> a = base_args<double>(e, e->s64, 0.0);
     a.tscal = d_ts.p;
     a.base = e->d_base + 1;

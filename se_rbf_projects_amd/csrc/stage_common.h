@@ -139,7 +139,4 @@ __device__ __forceinline__ void level_velocity(const StageArgs<T> &a, const T *B
 }
 
 
-template <typename T>
-void launch_tile(serbf_engine *e, int stage, int hv, bool xy, const StageArgs<T> &a);
-
 }  // namespace serbf

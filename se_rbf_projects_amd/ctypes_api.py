@@ -200,6 +200,15 @@ class PhysConstantsStruct(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("R", "g", "p0", "cp", "Rd", "kappa", "T0", "rho0", "htop")]
 
 
+class DomainsStruct(C.Structure):
+    _fields_ = [("Nnodes", C.c_int), ("Nparts", C.c_int), ("global_nodeset", NodesetStruct),
+                ("part_ids", c_int_p), ("part_sizes", c_int_p), ("part_start_ids", c_int_p)]
+
+
+class TC2ParamsStruct(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("tau", "K", "u0", "w0", "h1", "h2", "htop")]
+
+
 def lonlat(xyz):
     x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
     lam = np.arctan2(y, x)

@@ -68,7 +68,6 @@ def lib():
             "serbf_set_kernel_version": ([vp, i], i),
             "serbf_stream_copy": ([vp, C.c_size_t, i, vp], i),
             "serbf_set_stage_unroll": ([vp, i], i),
-            "serbf_set_tile_chunk": ([vp, i], i),
             "serbf_tile_info": ([vp, vp, vp], i),
             "serbf_get_lap_weights": ([vp, vp, vp], i),
             "serbf_set_stencil_sizes": ([vp, vp, vp], i),
@@ -358,7 +357,9 @@ class Engine:
         """RK4 steps with a history, as the reference prototype records q_hist
         (PY/Solver.ipynb:669-773): the state at t0 and after every `every` steps.  Steps past the
         last multiple of `every` are run but not recorded.  Returns times [H] and the states
-        [H, N, (tracers,) Nv]; the stepping between snapshots stays on the device."""
+        [H, N, (tracers,) Nv]; the stepping between snapshots stays on the device.  An attached
+        closure keeps its refresh cadence across the snapshots (the engine counts steps since
+        the state was set), so the snapshots and the final state equal one run of nsteps."""
         if int(every) < 1 or int(nsteps) < 0:
             raise ValueError("run_history: every >= 1 and nsteps >= 0")
         every, nsteps = int(every), int(nsteps)
@@ -391,11 +392,8 @@ class Engine:
     def set_kernel_version(self, v):
         """Stage kernel: 5 (default) matrix-core tiles, FP32 and FP64, 1 or 4 tracers of 64
         levels in config mode; 2 column kernel (every shape, and the fallback of the tile
-        kernels); 1, 3, 4, 6 other variants (see serbf.h)."""
+        kernel)."""
         _check(lib().serbf_set_kernel_version(self._e, int(v)), "set_kernel_version")
-
-    def set_tile_chunk(self, nbytes):
-        _check(lib().serbf_set_tile_chunk(self._e, int(nbytes)), "set_tile_chunk")
 
     def tile_info(self):
         nt, um = C.c_int(), C.c_int()

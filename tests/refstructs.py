@@ -72,3 +72,34 @@ def column_view(arr, N, stride, Nv, front=8):
 
 def arr(ptr, n, dtype=np.float64):
     return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def oracle_patch(xyz5, n, Nv, htop=12000.0, dt0=1800.0, weights=True):
+    """The oracle's struct-level sequence (orc_ref_*) on one patch holding every node:
+    stencils, layers, state arrays, Householder frames, weights (optional) and the TC2
+    initial state.  Returns (patch, views, rt, pc, own)."""
+    import oracle_lib as O
+
+    Ol = O.lib()
+    own = Owned()
+    N = len(xyz5)
+    p = T.PatchStruct()
+    fill_nodeset(p.nodeset, own, xyz5)
+    Ol.orc_ref_get_stencils(C.addressof(p.nodeset), n)
+    p.Nnodes = p.part_Nnodes = N
+    p.part_pids = own.int(np.arange(N))
+    pc = T.PhysConstantsStruct(R=6.37122e6, g=9.80616, p0=1.0e5, cp=1004.5, Rd=287.0, kappa=1004.5 / 287.0,
+                               T0=300.0, rho0=1.0e5 / (287.0 * 300.0), htop=htop)
+    rt = T.RtConfigStruct()
+    rt.num_layers, rt.model_height, rt.stencil_size, rt.TC, rt.timestep_length = Nv, htop, n, 2, dt0
+    Ol.orc_ref_get_layers(C.addressof(p.layers), htop, Nv, pc.R)
+    views = alloc_patch_state(p, own)
+    dm = p.rbffd_DMs
+    dm.part_Nnodes, dm.n = N, n
+    dm.idx, dm.D = p.nodeset.idx, p.nodeset.D
+    Ol.orc_ref_init_HH(C.addressof(p))
+    if weights:
+        Ol.orc_ref_get_rbffd_DMs(C.addressof(p), 5, 4, 0.044 * np.sqrt(N) - 0.14, pc.R)
+    Ol.orc_ref_init_TC2(C.addressof(p), C.addressof(rt), C.addressof(pc))
+    own.keep += [p, pc, rt]
+    return p, views, rt, pc, own

@@ -4,6 +4,7 @@ same sequence as the reference's main.c (one rank): get_stencils ->
 get_part_rbffd_stencils -> init_rbffd_HH_rot_Ms -> get_rbffd_DMs -> init_TC2 ->
 get_solution -> verify_solution."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -166,3 +167,60 @@ def test_dropin_get_solution_and_verify(runs):
     ref = np.empty(7)
     Ol.orc_ref_verify(C.addressof(o), N_NODES, C.addressof(runs["opc"]), ref.ctypes.data)
     assert np.allclose(got, ref, rtol=1e-8)
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "_bin", "dropin_main")
+
+
+def _driver():
+    """tests/dropin_main.c (main.c's sequence in C) linked against libserbf.so; built by
+    the package build, or here when missing."""
+    if not os.path.exists(DRIVER):
+        import subprocess
+        root = os.path.dirname(os.path.dirname(DRIVER))
+        root = os.path.dirname(root)
+        os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+        subprocess.run(["gcc", "-O2", "-std=gnu11", "-I", os.path.join(root, "include"),
+                        os.path.join(root, "tests", "dropin_main.c"), "-L", os.path.join(root, "se_rbf_projects_amd"),
+                        "-lserbf", "-Wl,-rpath,$ORIGIN/../../se_rbf_projects_amd", "-o", DRIVER], check=True)
+    return DRIVER
+
+
+def test_c_driver_runs_main_sequence(golden, tmp_path):
+    """A C program with main.c's call order (main.c:22-67 without MPI) linked against
+    libserbf.so alone, on the reference's own ICOS_642.nc: its verify_solution banner
+    against the oracle's struct-level sequence on the same node set, and against the
+    reference's own verify_solution (built from its unmodified wrappers.c, when present)
+    run on the driver's final state."""
+    import subprocess
+
+    import refharness as RH
+
+    env = {k: v for k, v in os.environ.items() if not k.startswith("ADV_")}
+    env.update(RH.adv_env(nodeset_file=os.path.join(golden.path, "ICOS_642.nc"), num_layers=NV,
+                          stencil_size=NSTEN, model_height=HTOP, timestep=DT0))
+    qfile = str(tmp_path / "q.bin")
+    r = subprocess.run([_driver(), qfile], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    assert out.count("Solver Status:") == 48 and "tstep = 47 -> t = 84600.0 s" in out
+    got = RH.parse_verification(out)
+    q = np.fromfile(qfile).reshape(N_NODES, NV)
+    # the oracle's struct-level sequence on the same nodes
+    p, views, rt, pc, own = RS.oracle_patch(golden.icos(N_NODES), NSTEN, NV, HTOP, DT0)
+    O.lib().orc_ref_get_solution(C.addressof(p), C.addressof(rt), C.addressof(pc))
+    oq = RS.column_view(views["SV_q_t"], N_NODES, p.layers.padded_Nvt, NV)
+    assert np.linalg.norm(q - oq) / np.linalg.norm(oq) < 1e-10
+    ref = np.empty(7)
+    O.lib().orc_ref_verify(C.addressof(p), N_NODES, C.addressof(pc), ref.ctypes.data)
+    names = ("L1", "L2", "Linf", "mass_real", "mass_approx", "min", "max")
+    for k, v in zip(names, ref):
+        assert abs(got[k] - v) <= 1e-6 * abs(v), (k, got[k], v)
+    if RH.available():
+        xyz5 = golden.icos(N_NODES)
+        idx, D = O.knn(xyz5[:, :3], NSTEN)
+        res = RH.run(xyz5, idx, D, RH.adv_env(num_layers=NV, stencil_size=NSTEN, model_height=HTOP, timestep=DT0),
+                     qt=q)
+        want = RH.parse_verification(res.stdout)
+        for k in names:
+            assert abs(got[k] - want[k]) <= 1e-6 * abs(want[k]), (k, got[k], want[k])
