@@ -15,14 +15,18 @@ Timed legs on the same inputs:
                closure's per-point coefficients (re-evaluated every 128 steps)  -> value
   f32          the same with the constant coefficient (its stage kernel gives the roofline)
 Both precisions use the matrix-core tile stage kernel (kernel version 5).
---period runs both the f64 and the f32+closure legs over the full 12-day period of the
-reversing flow and states their error against the exact solution q(T) = q(0), together with
-FP32 at the constant coefficient and at s_max times it.
+After the timed legs, the f64 and f32+closure legs run over the full 12-day period of the
+reversing flow (about 7000 steps each, untimed for `value`) and the line states their error
+against the exact solution q(T) = q(0) (--no-period skips this; --period-sweep adds FP32 at
+constant coefficients for comparison with the closure).
 Secondary legs (reported beside the headline, not in it): the uneven batched weight solves at
-the same level (config C), and the plain FP64 method over a full period one level lower
-(163842 nodes x 60, one field of two Gaussian hills; config B).
+the same level with a LAPACK residual on sampled nodes (config C), and the plain FP64 method
+over a full period one level lower (163842 nodes x 60, one field of two Gaussian hills;
+config B).  The CPU baseline is the oracle built here at -O3 -march=native, timed on all host
+cores over a bounded sample of the workload, plus one thread on the CPU-runnable case (2562
+nodes x 10 levels, 100 steps).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--period]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-period] [--period-sweep]
 """
 import argparse
 import json
@@ -104,12 +108,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="RK4 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=1024, help="size of the stream-copy bandwidth reference")
-    ap.add_argument("--period", action="store_true", help="also run the full period and state the errors")
+    ap.add_argument("--no-period", action="store_true", help="skip the full-period error legs")
+    ap.add_argument("--period-sweep", action="store_true",
+                    help="with the period legs, also FP32 at constant coefficients (1, s_max and the multipliers "
+                         "given by --sweep-multipliers)")
+    ap.add_argument("--sweep-multipliers", type=str, default="", help="comma-separated extra constant multipliers")
     ap.add_argument("--closure-every", type=int, default=128, help="steps between closure re-evaluations")
     ap.add_argument("--no-config-c", action="store_true", help="skip the uneven batched weight-solve leg")
     ap.add_argument("--no-config-b", action="store_true", help="skip the FP64 full-period leg at one level lower")
-    ap.add_argument("--kernel-version", type=int, default=5,
-                    help="stage kernel: 5 matrix-core tiles (default), 2 column kernel, 4/6 tile variants")
+    ap.add_argument("--kernel-version", type=int, default=5, choices=(2, 5),
+                    help="stage kernel: 5 matrix-core tiles (default), 2 column kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not part of "
@@ -181,8 +189,9 @@ def main():
     achieved64 = b64 / (res["f64"][1] / 1e3) / 1e9
 
     period = None
-    if args.period:
-        period = {"steps": nsteps_T, "dt_s": dt, "reference": "exact solution q(T) = q(0) of the reversing flow"}
+    if not args.no_period:
+        period = {"steps": nsteps_T, "dt_s": dt, "reference": "exact solution q(T) = q(0) of the reversing flow",
+                  "norms": "relative l2 / linf over the three transported tracers"}
         for name, prec, split, use_cl in legs[:2]:
             eng.set_hv_split(split)
             eng.attach_closure(closure if use_cl else None, every=args.closure_every, s_max=s_max)
@@ -194,27 +203,25 @@ def main():
             period[name]["mass_const_tracer"] = float(np.mean(q[:, 3]) - 1.0)
             period[name]["seconds"] = time.time() - t
             log(f"period {name}: {period[name]}")
-        eng.set_hv_split(False)
-        eng.attach_closure(None)
-        eng.set_state(q0)
-        eng.run(F32, nsteps_T, dt)
-        period["f32 constant coefficient"] = error_vs(eng.state(F32), q0)
-        log(f"period f32 constant coefficient: {period['f32 constant coefficient']}")
-        # the closure's multiplier is mostly at its clamp: the constant coefficient times s_max
-        # separates what the learned field adds from "damp harder everywhere"
-        cfg16 = config_deform(N, Nv=args.nv, n=args.n, deg=4, nv=7)
-        cfg16.ntracers = ntr
-        cfg16.hv_coef = cfg.hv_coef * s_max
-        eng.configure(cfg16)
-        eng.weights()
-        eng.set_state(q0)
-        eng.run(F32, nsteps_T, dt)
-        key16 = f"f32 constant coefficient x {s_max:g}"
-        period[key16] = error_vs(eng.state(F32), q0)
-        log(f"period {key16}: {period[key16]}")
-        eng.configure(cfg)
-        eng.weights()
-        eng.set_state(q0)
+        period["f32+closure_vs_f64_l2"] = period["f32+closure"]["rel_l2"] / period["f64"]["rel_l2"]
+        if args.period_sweep:
+            eng.set_hv_split(False)
+            eng.attach_closure(None)
+            mults = [1.0, s_max] + [float(m) for m in args.sweep_multipliers.split(",") if m.strip()]
+            for mlt in mults:
+                cfgm = config_deform(N, Nv=args.nv, n=args.n, deg=4, nv=7)
+                cfgm.ntracers = ntr
+                cfgm.hv_coef = cfg.hv_coef * mlt
+                eng.configure(cfgm)
+                eng.weights()
+                eng.set_state(q0)
+                eng.run(F32, nsteps_T, dt)
+                key = f"f32 constant coefficient x {mlt:g}"
+                period[key] = error_vs(eng.state(F32), q0)
+                log(f"period {key}: {period[key]}")
+            eng.configure(cfg)
+            eng.weights()
+            eng.set_state(q0)
     eng.attach_closure(None)
     eng.set_hv_split(False)
 
@@ -254,10 +261,8 @@ def main():
                    "nodes": N, "levels": cfg.Nv, "tracers": ntr, "stencil": cfg.n, "dt_s": dt,
                    "point_update": "one (node, level) advanced one RK4 step (all 4 tracers)",
                    "closure": closure_kind, "closure_every_steps": args.closure_every, "closure_s_max": s_max,
-                   "stage_kernel": {2: "column kernel (version 2)", 4: "matrix-core tiles staged in LDS (version 4)",
-                                    5: "matrix-core tiles (version 5)",
-                                    6: "matrix-core tile pipeline (version 6)"}.get(args.kernel_version,
-                                                                                    f"version {args.kernel_version}")
+                   "stage_kernel": {2: "column kernel (version 2)",
+                                    5: "matrix-core tiles (version 5)"}[args.kernel_version]
                    + " in both precisions where it applies"},
         "fp64_plain": {"value": value["f64"], "ms_per_step": res["f64"][0] / args.steps,
                        "stage_us": res["f64"][1] * 1e3, "roofline_achieved_GBs": achieved64,
@@ -267,9 +272,11 @@ def main():
         "tracer_point_updates_per_s": value["f32+closure"] * ntr,
         "error": {"after_steps": args.steps, "vs_f64_leg": short, "period": period},
         "stream_copy_GBs": copy_gbs,
+        "stage_frac_of_copy": achieved / copy_gbs,
         "setup": {"knn_s": knn_s, "weights_s": w_s, "weight_solves_per_s": 2 * N / w_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac_of_tuned_copy": achieved / copy_gbs,
                      "kernel": ("mtile2_kernel<float, STAGE, 1, 2> (matrix-core tiles)" if args.kernel_version == 5
                                 else f"stage kernel version {args.kernel_version}")
                                + ", average of the 4 launches of a step",
@@ -279,7 +286,7 @@ def main():
                      "mfma_view": {"dense_flops_per_launch": dense_fl, "useful_flops_per_launch": useful_fl,
                                    "mean_tile_union": umean, "achieved_TFLOPs": mfma_tf,
                                    "peak_TFLOPs": MFMA_F32_PEAK_TFLOPS, "frac": mfma_tf / MFMA_F32_PEAK_TFLOPS}
-                     if args.kernel_version in (4, 5, 6) else None},
+                     if args.kernel_version == 5 else None},
         "cpu_baseline": cpu,
         "config_c_weights": weights_c,
         "config_b_fp64_period": period_b,
@@ -311,6 +318,8 @@ def config_c_weights(E, level):
     tune_s = time.time() - t
     eng.weights()
     ms, systems, v = eng.weights_stats()
+    Dx, Dy, _, st = eng.get_weights()
+    lapack = lapack_residuals(xyz, eng.stencils()[0], nn, deg, Dx, Dy, samples=256)
     M = (deg + 1) * (deg + 2) // 2
     m = (nn + M).astype(np.float64)
     flops = float(np.sum(2.0 / 3.0 * m ** 3 + 2 * m ** 2 * 4 + 4 * m ** 2 * 4)
@@ -319,7 +328,10 @@ def config_c_weights(E, level):
     out = {"nodes": N, "stencil_sizes": [int(nn.min()), int(nn.max())], "degrees": sorted(set(deg.tolist())),
            "systems": systems, "kernel_ms": ms, "solves_per_s": systems / (ms * 1e-3), "variant": v,
            "autotune": [{"variant": a, "ms": b, "solves_per_s": c} for a, b, c in rep], "autotune_s": tune_s,
-           "fp64_tflops": flops / (ms * 1e-3) / 1e12}
+           "fp64_tflops": flops / (ms * 1e-3) / 1e12, "failed_systems": int(np.count_nonzero(st)),
+           "vs_lapack": lapack}
+    log(f"config C vs LAPACK on {lapack['sampled_nodes']} sampled nodes: max weight difference "
+        f"{lapack['max_weight_rel_diff']:.2e}, max scaled residual {lapack['max_scaled_residual']:.2e}")
     log(f"config C weights: {systems:.0f} systems in {ms:.1f} ms ({out['solves_per_s']:.3e}/s, variant {v}, "
         f"{out['fp64_tflops']:.2f} TFLOP/s FP64)")
     return out
@@ -362,34 +374,127 @@ def config_b_period(E, level, kernel_version):
     return out
 
 
+def householder(p):
+    """The reflector H = I - beta v v^T with H p = e_z (the stable branch for z > 0), and the
+    frame diag(1, -1, -1) at the south pole, as the weight solves use in config mode."""
+    x, y, z = p
+    lon = x * x + y * y
+    if lon == 0.0:
+        return np.eye(3) if z > 0 else np.diag([1.0, -1.0, -1.0])
+    v2 = -lon / (z + 1.0) if z > 0.0 else z - 1.0
+    beta = 2.0 * v2 * v2 / (lon + v2 * v2)
+    v = np.array([x / v2, y / v2, 1.0])
+    return np.eye(3) - beta * np.outer(v, v)
+
+
+def lapack_residuals(xyz, idx, nn, deg, Dx, Dy, samples=256, phs=7):
+    """The config-mode gradient systems of sampled nodes (PHS r^phs + degree-d tangent
+    monomials on the node's rotated, centred stencil scaled by its radius) assembled in numpy
+    and solved with numpy.linalg.solve (LAPACK dgesv): the largest difference of the GPU
+    weights from LAPACK's, and the largest scaled residual ||A w - b|| / (||A|| ||w||) of the
+    GPU weights in the same systems (polynomial part completed by least squares)."""
+    N = len(Dx)
+    worst_w, worst_r, count = 0.0, 0.0, 0
+    for p in np.linspace(0, N - 1, samples).astype(int):
+        n, d = int(nn[p]), int(deg[p])
+        X = (householder(xyz[idx[p, 0]]) @ xyz[idx[p, :n]].T).T
+        X = X - X[0]
+        s = np.max(np.linalg.norm(X, axis=1))
+        X = X / s
+        r = np.linalg.norm(X[:, None, :] - X[None, :, :], axis=2)
+        mons = [(q - j, j) for q in range(d + 1) for j in range(q + 1)]
+        P = np.stack([X[:, 0] ** a * X[:, 1] ** b for a, b in mons], axis=1)
+        M = len(mons)
+        A = np.zeros((n + M, n + M))
+        A[:n, :n] = r ** phs
+        A[:n, n:] = P
+        A[n:, :n] = P.T
+        r0 = np.linalg.norm(X, axis=1)
+        b = np.zeros((n + M, 2))
+        b[:n, 0] = phs * (-X[:, 0]) * r0 ** (phs - 2)
+        b[:n, 1] = phs * (-X[:, 1]) * r0 ** (phs - 2)
+        b[n + mons.index((1, 0)), 0] = 1.0
+        b[n + mons.index((0, 1)), 1] = 1.0
+        ref = np.linalg.solve(A, b)
+        got = np.zeros_like(ref)
+        got[:n, 0] = Dx[p, :n] * s
+        got[:n, 1] = Dy[p, :n] * s
+        worst_w = max(worst_w, float(np.max(np.abs(got[:n] - ref[:n])) / np.max(np.abs(ref[:n]))))
+        got[n:] = np.linalg.lstsq(A[:, n:], b - A[:, :n] @ got[:n], rcond=None)[0]
+        worst_r = max(worst_r, float(np.linalg.norm(A @ got - b) / (np.linalg.norm(A) * np.linalg.norm(got))))
+        count += 1
+    return {"sampled_nodes": count, "max_weight_rel_diff": worst_w, "max_scaled_residual": worst_r,
+            "systems": "gradient systems (PHS r^7 + tangent monomials), numpy.linalg.solve"}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(eng, cfg, xyz, q0, dt, steps):
-    """The oracle's FP64 RK4 (C + OpenMP over nodes) on the same stencils and
-    weights, for a bounded number of steps on the host cores."""
+    """The oracle's FP64 RK4 (C + OpenMP over nodes), compiled here at -O3 -march=native,
+    on the same stencils and weights for a bounded number of steps on all host cores, and on
+    one core over the CPU-runnable case (2562 nodes x 10 levels, n = 31, 100 steps)."""
+    import subprocess
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native"], check=True,
+                       capture_output=True, text=True)
+        native = os.path.join(ROOT, "oracle", "_ref", "liboracle_native.so")
+        build = "gcc -O3 -march=native -fopenmp -ffp-contract=off"
         import oracle_lib as O
+        Ol = O.lib(native)
     except Exception as ex:  # pragma: no cover
-        log(f"cpu baseline unavailable: {ex}")
-        return None
+        log(f"native oracle build unavailable ({ex}); timing the shipped build")
+        native, build = None, "gcc -O3 -fopenmp -ffp-contract=off"
+        import oracle_lib as O
+        Ol = O.lib()
+    from se_rbf_projects_amd.ctypes_api import F64, config_A, courant_dt
+    cores = len(os.sched_getaffinity(0))
+    Ol.orc_set_threads(cores)
+    threads = Ol.orc_max_threads()
     idx, _ = eng.stencils()
     Dx, Dy, L, _ = eng.get_weights()
     P = O.Problem(cfg, xyz, idx, Dx, Dy, L)
-    cores = len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
     t = time.time()
     q_cpu = P.rk4(q0, steps, dt)
     s = time.time() - t
-    from se_rbf_projects_amd.ctypes_api import F64
     eng.set_hv_split(False)
     eng.set_state(q0)
     eng.run(F64, steps, dt)
     parity = float(np.linalg.norm(eng.state(F64) - q_cpu) / np.linalg.norm(q_cpu))
     log(f"full-scale parity, GPU FP64 vs oracle after {steps} steps: rel l2 {parity:.3e}")
     pts = len(xyz) * cfg.Nv
-    log(f"cpu baseline: {steps} steps in {s:.2f}s on {cores} cores")
-    return {"value": pts * steps / s, "unit": "point-updates/s", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
-            "kind": "port", "sample": f"{steps} RK4 steps of the full workload (FP64 oracle, OpenMP over nodes)",
-            "cpu": platform.processor() or platform.machine(), "gpu_f64_vs_oracle_rel_l2": parity}
+    log(f"cpu baseline: {steps} steps in {s:.2f}s on {threads} threads")
+    # one thread on the CPU-runnable case
+    from se_rbf_projects_amd import engine as E
+    xa = E.icos_nodes(4)
+    ca = config_A(len(xa))
+    ia, _ = O.knn(xa, ca.n)
+    wa = O.weights(ca, xa, ia)
+    qa = O.initial(E.IC_BELL, ca, xa)
+    dta = courant_dt(ca, len(xa))
+    Ol.orc_set_threads(1)
+    t = time.time()
+    O.Problem(ca, xa, ia, wa[0], wa[1], wa[2]).rk4(qa, 100, dta)
+    s1 = time.time() - t
+    Ol.orc_set_threads(threads)
+    log(f"cpu baseline, 1 thread, 2562 x 10 x 100 steps: {s1:.2f}s")
+    return {"value": pts * steps / s, "unit": "point-updates/s", "cores": threads,
+            "kind": "port", "sample": f"{steps} RK4 steps of the full workload (FP64 oracle, OpenMP over nodes, "
+                                      f"{threads} threads)",
+            "cpu": cpu_model(), "build": build,
+            "one_thread_config_A": {"value": len(xa) * ca.Nv * 100 / s1, "unit": "point-updates/s",
+                                    "sample": f"{len(xa)} nodes x {ca.Nv} levels, n={ca.n}, 100 RK4 steps, "
+                                              "FP64, 1 thread", "seconds": s1},
+            "gpu_f64_vs_oracle_rel_l2": parity}
 
 
 if __name__ == "__main__":

@@ -91,6 +91,10 @@ void orc_rk4(const serbf_config *cfg, int N, const double *x, const double *y, c
              const int *vstart, const double *vW, int nsteps, double dt, double t0,
              double *q);
 
+/* OpenMP team size of the parallel loops */
+void orc_set_threads(int n);
+int orc_max_threads(void);
+
 /* {L1, L2, Linf, mass_ref, mass, min, max}; Linf denominator max|q| (wrappers.c:152) */
 void orc_norms(const serbf_config *cfg, int N, const double *q_ref, const double *q,
                double out[7]);

@@ -12,9 +12,19 @@
  * expression: F = F1; F = F + 2F2; F = F + 2F3; F = F + 1.0F4; q = q + dt/6 F.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 #include "oracle.h"
+
+/* OpenMP team size of the oracle's parallel loops (the CPU-baseline timing sets and
+ * reports it) */
+void orc_set_threads(int n)
+{
+    if (n > 0)
+        omp_set_num_threads(n);
+}
+int orc_max_threads(void) { return omp_get_max_threads(); }
 
 static double rho_of(double h)
 {

@@ -8,7 +8,7 @@
 `rocprofv3 --kernel-trace --stats` run; `pmc` prints per-kernel averages of the
 counters of one or more `rocprofv3 --pmc ...` runs; `traffic` turns a FETCH_SIZE run and
 a WRITE_SIZE run into HBM bytes per launch, each counter scaled by what it reads for the
-16-byte stream-copy kernel of the same run (which moves copy-MiB in each direction).  Both read the rocpd SQLite
+plain 16-byte stream-copy kernel of the same run (which moves copy-MiB in each direction).  Both read the rocpd SQLite
 database rocprofv3 writes (the `kernels` / `counters_collection` views).
 """
 import collections
@@ -87,7 +87,8 @@ def traffic(fetch_dir, write_dir, copy_mib):
     stream_copy_kernel, and the calibration factors."""
     f = counter_stats([fetch_dir])
     w = counter_stats([write_dir])
-    copy = "stream_copy_kernel"
+    # the plain 16-byte copy (one vector per thread, ordinary loads and stores)
+    copy = next(k for k in ("stream_copy_kernel<1, false>", "stream_copy_kernel") if k in f and k in w)
     nbytes = copy_mib * 1024 * 1024
     kf = nbytes / (f[copy]["FETCH_SIZE"] * 1024.0)
     kw = nbytes / (w[copy]["WRITE_SIZE"] * 1024.0)

@@ -1,14 +1,40 @@
 """FP32 error bars of the GPU parity tests.
 
-Each bar sits about ten times above the relative error measured on an MI355X for
-the same case (the measured value is written beside the bar), so a regression of
-an order of magnitude fails.  With SERBF_F32_LOG=<file> set, every check appends
-its measured error to that file as a JSON line."""
+Each bar sits about ten times above the relative l2 error measured on an MI355X for
+the same case (written beside it), so a regression of an order of magnitude fails.
+The cases at 2562 nodes with n = 31 carry the Gaussian Delta^4 weights' cancellation
+(the n = 31 stencil's weights are large and cancel strongly); their bars sit six to
+seven times above the measured error.  With SERBF_F32_LOG=<file> set, every check
+appends its measured error to that file as a JSON line."""
 import json
 import os
 
+# case: (bar, measured)
+BARS = {
+    "config_D_shape_v5": (1e-6, 8.43e-8),
+    "config_D_shape_v2": (1e-6, 6.67e-8),
+    "config_D_shape_v5_no_hv": (6e-7, 5.27e-8),
+    "config_D_closure_fp16": (1e-6, 8.38e-8),
+    "config_A_100_steps": (1.2e-5, 1.09e-6),
+    "four_tracers_642x16": (1.2e-6, 1.07e-7),
+    "levels64_v2_ntr4": (1e-4, 1.72e-5),
+    "levels64_v5_ntr4": (1e-4, 1.67e-5),
+    "levels64_v5_ntr4_vs_column": (2e-4, 2.78e-5),
+    "levels64_v2_ntr1": (5e-5, 4.34e-6),
+    "levels64_v5_ntr1": (6e-5, 5.95e-6),
+    "levels64_v5_ntr1_vs_column": (7e-5, 7.12e-6),
+    "deform_10242x32": (1.5e-6, 1.25e-7),
+    "run_history_f32_2562x60x4": (1e-5, 8.83e-7),
+    "hv_split_config_A": (2e-6, 1.84e-7),
+    # not yet measured on the GPU: the earlier bars
+    "uneven_time_loop_f32": (1e-4, None),
+    "uneven_tile_vs_column_f32": (1.2e-6, 1.2e-7),
+    "closure_rk4_642x10_f32": (1e-4, None),
+}
 
-def check(name, err, bar):
+
+def check(name, err):
+    bar, _ = BARS[name]
     log = os.environ.get("SERBF_F32_LOG")
     if log:
         with open(log, "a") as f:

@@ -19,12 +19,16 @@ def build():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 
 
-def lib():
+def lib(path=None):
+    """The oracle library; `path` loads another build of the same sources (the CPU-baseline
+    timing's -march=native build) in place of oracle/liboracle.so, before the first use."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if path is None and not os.path.exists(LIB_PATH):
             build()
-        _lib = C.CDLL(LIB_PATH)
+        _lib = C.CDLL(path or LIB_PATH)
+        _lib.orc_max_threads.restype = C.c_int
+        _lib.orc_set_threads.argtypes = [C.c_int]
         _lib.orc_householder.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, c_double_p]
         _lib.orc_dgesv.restype = C.c_int
         _lib.orc_weights.restype = C.c_int

@@ -40,3 +40,28 @@ def test_more_than_one_gpu_is_refused():
                        text=True, timeout=120, cwd=root)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_bench_householder_and_lapack_residuals(golden):
+    """The bench's numpy frame maps each node to the pole like the oracle's; its LAPACK check
+    of the config-mode gradient systems finds the oracle's weights (its own LU) within
+    rounding, and finds a perturbed weight set."""
+    import oracle_lib as O
+    from se_rbf_projects_amd.ctypes_api import config_deform, variable_stencil_sizes
+
+    xyz = golden.icos(642)[:, :3]
+    for p in (xyz[5], xyz[100], np.array([0.0, 0.0, -1.0]), np.array([0.0, 0.6, 0.8])):
+        H = bench.householder(p)
+        assert np.allclose(H @ p, [0, 0, 1], atol=1e-15) and np.allclose(H @ H.T, np.eye(3), atol=1e-15)
+        assert np.allclose(H, O.householder(p, fix_pole=True), atol=1e-15)
+    cfg = config_deform(642, Nv=10, n=40, deg=4, nv=5)
+    idx, D = O.knn(xyz, 40)
+    nn, deg = variable_stencil_sizes(D, n_min=31, n_max=40)
+    Dx, Dy, L, st = O.weights_var(cfg, xyz, idx, nn, deg)
+    assert not st.any()
+    r = bench.lapack_residuals(xyz, idx, nn, deg, Dx, Dy, samples=40)
+    assert r["sampled_nodes"] == 40 and r["max_weight_rel_diff"] < 1e-8 and r["max_scaled_residual"] < 1e-13
+    Dx2 = Dx.copy()
+    Dx2[::7, 3] *= 1.001
+    r2 = bench.lapack_residuals(xyz, idx, nn, deg, Dx2, Dy, samples=40)
+    assert r2["max_weight_rel_diff"] > 1e-5

@@ -4,6 +4,7 @@ network gains over the constant coefficient."""
 import numpy as np
 import pytest
 
+import fp32bars
 import oracle_lib as O
 from se_rbf_projects_amd import closure as CL
 from se_rbf_projects_amd import engine as E
@@ -207,7 +208,7 @@ def test_rk4_with_closure_matches_oracle(golden):
     assert err < 1e-11
     eng.set_state(q0)
     eng.run(F32, 8, dt)
-    assert np.linalg.norm(eng.state(F32) - q64) / np.linalg.norm(q64) < 1e-4
+    fp32bars.check("closure_rk4_642x10_f32", np.linalg.norm(eng.state(F32) - q64) / np.linalg.norm(q64))
     eng.attach_closure(None)
     cl.close()
     eng.close()
