@@ -26,10 +26,9 @@ BARS = {
     "deform_10242x32": (1.5e-6, 1.25e-7),
     "run_history_f32_2562x60x4": (1e-5, 8.83e-7),
     "hv_split_config_A": (2e-6, 1.84e-7),
-    # not yet measured on the GPU: the earlier bars
-    "uneven_time_loop_f32": (1e-4, None),
-    "uneven_tile_vs_column_f32": (1.2e-6, 1.2e-7),
-    "closure_rk4_642x10_f32": (1e-4, None),
+    "uneven_time_loop_f32": (1.6e-6, 1.58e-7),
+    "uneven_tile_vs_column_f32": (1.2e-6, 1.20e-7),
+    "closure_rk4_642x10_f32": (1.6e-6, 1.60e-7),
 }
 
 
