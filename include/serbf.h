@@ -151,11 +151,17 @@ int serbf_weights(serbf_engine *e);
  * All the uneven systems are solved in one persistent launch by serbf_weights.  nn = deg = NULL returns to uniform stencils.  Call after
  * serbf_configure. */
 int serbf_set_stencil_sizes(serbf_engine *e, const int *nn, const int *deg);
-/* Autotuner of the batched weight solves: times each schedule variant that fits (refinement
- * residual re-evaluated or from an LDS copy of the matrix; node order or largest systems
- * first) on `sample` evenly spaced nodes and keeps the fastest for serbf_weights.
- * report[3*v + {0,1,2}] = variant, ms (-1: does not fit), systems per second; *nvariants = 4. */
+/* Autotuner of the batched weight solves: times each schedule variant that fits on `sample`
+ * evenly spaced nodes and keeps the fastest for serbf_weights.  Variants 0-3: the LDS kernel
+ * (one wavefront per system, rows in LDS) with the refinement residual re-evaluated (0, 1) or
+ * read from an LDS copy of the matrix (2, 3); 4-5 (4 the default): the register-row kernel
+ * (one wavefront per system, one or two rows per lane in registers, one persistent launch per
+ * size class and system kind); even variants in node order, odd ones largest systems first.
+ * report[3*v + {0,1,2}] = variant, ms (-1: does not fit), systems per second; *nvariants = 6
+ * (report holds at least 48 doubles). */
 int serbf_weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants);
+/* Pin the schedule variant of serbf_weights (0-5, as the autotuner numbers them). */
+int serbf_weights_set_variant(serbf_engine *e, int variant);
 /* The last serbf_weights launch: kernel time (ms), systems solved, variant used. */
 int serbf_weights_stats(serbf_engine *e, double *ms, double *systems, int *variant);
 /* status[i] = 0 ok, 1 singular gradient system, 2 singular Delta^k system */

@@ -115,7 +115,7 @@ struct serbf_engine {
     // per-node stencil sizes / polynomial degrees (empty: uniform), the weight-solve schedule
     std::vector<int> h_nn, h_deg;
     int *d_nn = nullptr, *d_deg = nullptr, *d_worder = nullptr;
-    int wvariant = 0;            // schedule variant of the weight solves (autotuner)
+    int wvariant = 4;            // schedule variant of the weight solves (autotuner; 4: register rows)
     double wstats_ms = 0.0, wstats_systems = 0.0;
     int wstats_variant = -1;
     int *d_status = nullptr;

@@ -441,6 +441,13 @@ int serbf_weights_autotune(serbf_engine *e, int sample, double *report, int *nva
     return weights_autotune(e, sample, report, nvariants);
 }
 
+int serbf_weights_set_variant(serbf_engine *e, int variant)
+{
+    SERBF_CHECK(e && variant >= 0 && variant < 6, SERBF_ERR_ARG, "weights_set_variant: 0 to 5");
+    e->wvariant = variant;
+    return SERBF_OK;
+}
+
 int serbf_weights_stats(serbf_engine *e, double *ms, double *systems, int *variant)
 {
     SERBF_CHECK(e && e->wstats_variant >= 0, SERBF_ERR_STATE, "weights_stats: no weight solve yet");

@@ -1,14 +1,16 @@
 // Batched per-node RBF-FD weight solves (gfx950), the body of get_rbffd_DMs
 // (rbffd_DMs.c:98-282 of the reference).
 //
-// One persistent launch factors every node's saddle system.  A wavefront owns one
-// system at a time: it gathers the stencil, rotates it with the node's
-// Householder reflector, assembles [Phi P; P^T 0] and the right-hand sides in LDS
-// (lane = row, rows lane + 64 r), factors it with partial pivoting (first maximal
-// |a_ik|, full-row swaps as LAPACK dgesv), solves, and applies one step of
-// iterative refinement whose residual b - A x re-evaluates A from the
-// coordinates (the matrix itself is never stored in HBM).  Work is pulled from a
-// global counter so uneven system sizes balance across the chip.
+// Persistent launches factor every node's saddle systems; work is pulled from a global counter
+// so uneven system sizes balance across the chip.  Two kernels:
+//   - the register-row kernel (rowlu.h, the default schedule): one work-group per system,
+//     one row of [A | B] per thread in registers; one launch per size class and system kind;
+//   - the LDS kernel below: one wavefront per node, which assembles [Phi P; P^T 0] and the
+//     right-hand sides in LDS (lane = row, rows lane + 64 r), factors it with partial pivoting
+//     (first maximal |a_ik|, full-row swaps as LAPACK dgesv) and solves both of the node's
+//     systems; it takes the systems larger than the register classes.
+// Both apply one step of iterative refinement whose residual b - A x re-evaluates A from the
+// coordinates (the matrix itself is never stored in HBM).
 //
 // Systems per node:
 //   reference mode: PHS r^k (k = phs_order) + degree-5 tangent monomials, RHS
@@ -26,59 +28,10 @@
 #include <cstring>
 #include <vector>
 
-#include "device_util.h"
-#include "engine.h"
+#include "rowlu.h"
+#include "weights_common.h"
 
 namespace serbf {
-
-struct WArgs {
-    int N, n, ref_mode;
-    int phs, deg, hv, hvphs;
-    double eps;
-    const double *x, *y, *z;
-    const int *idx;
-    double *Wx, *Wy, *WL;
-    double *WL1, *WL2;   // config mode: Delta and Delta^2 weights from the gradient system (may be null)
-    int *status;
-    int *counter;
-    const int *nn, *dg;  // per-node stencil size and polynomial degree (null: n, deg for every node)
-    const int *order;    // work item -> node (null: identity)
-    int count;           // work items
-};
-
-// x^k for a small integer k >= 0 by repeated squaring (the PHS orders and monomial degrees)
-__device__ __forceinline__ double ipow(double x, int k)
-{
-    double r = 1.0, b = x;
-    while (k > 0) {
-        if (k & 1) r *= b;
-        b *= b;
-        k >>= 1;
-    }
-    return r;
-}
-__device__ __forceinline__ double phs_pow(double r, int k) { return k >= 0 ? ipow(r, k) : pow(r, (double)k); }
-
-// 2D Delta^k of the Gaussian exp(-(eps r)^2) by the Laguerre recursion (rbffd_DMs.c:314-328)
-__device__ double lap_gauss(double r, double eps, int k)
-{
-    double epr2 = pow(eps * r, 2.0);
-    double pm = 1.0, p = 4 * epr2 - 4;
-    if (k == 0) p = 1.0;
-    for (int i = 1; i < k; i++) {
-        double pn = 4 * (epr2 - (2 * i + 1)) * p - (16 * pow((double)i, 2.0)) * pm;
-        pm = p;
-        p = pn;
-    }
-    return pow(eps, 2.0 * k) * p * exp(-pow(r * eps, 2.0));
-}
-
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // LDS layout of one wavefront's system (doubles):
 //   A   [MAXM][LD]  augmented matrix [A | B] (LD >= MAXM + 4, odd)
@@ -86,8 +39,6 @@ __device__ __forceinline__ void wave_sync()
 //   B0  [MAXM][4]   right-hand sides (kept for the residual)
 //   XS  [MAXM][4]   solution
 //   piv [MAXM]      pivot rows (int)
-// right-hand-side columns per system (gradient system: d/dx', d/dy', Delta, Delta^2)
-constexpr int kRhs = 4;
 
 //   A0  [MAXM][MAXM] (COPY variant) the assembled matrix, for the refinement residual
 template <int MAXM, bool COPY>
@@ -97,30 +48,6 @@ struct Lds {
     static constexpr int doubles = MAXM * LD + 3 * MAXN + 2 * MAXM * kRhs + (COPY ? MAXM * MAXM : 0);
     static constexpr int bytes = doubles * 8 + MAXM * 4;
 };
-
-// Matrix entry A_ij of the current system (rows/cols < m); kind 0: gradient PHS system,
-// kind 1: Delta^k system, Gaussian + constant (reference mode, or config mode with
-// hv_phs_order 0; a.eps is the shape parameter in the solve frame) or PHS r^K + monomials.
-__device__ __forceinline__ double entry(int i, int j, int n, int m, int kind, const WArgs &a, const double *X,
-                                        int MAXN)
-{
-    const bool gauss = kind == 1 && (a.ref_mode || a.hvphs == 0);
-    if (i < n && j < n) {
-        const double dx = X[j] - X[i], dy = X[MAXN + j] - X[MAXN + i], dz = X[2 * MAXN + j] - X[2 * MAXN + i];
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        if (gauss) return exp(-pow(r * a.eps, 2.0));
-        return phs_pow(r, kind == 0 ? a.phs : a.hvphs);
-    }
-    if (i >= n && j >= n) return 0.0;
-    const int node = i < n ? i : j;
-    const int t = i < n ? j - n : i - n;
-    if (gauss) return 1.0;
-    // monomial t in the order x^(p-q) y^q, p = 0..deg, q = 0..p
-    int p = 0;
-    while ((p + 1) * (p + 2) / 2 <= t) p++;
-    const int q = t - p * (p + 1) / 2;
-    return ipow(X[node], p - q) * ipow(X[MAXN + node], q);
-}
 
 template <int MAXM, bool COPY>
 __device__ int factor_solve(double *A, double *X, double *B0, double *XS, const double *A0, int *piv, int n, int m,
@@ -281,41 +208,8 @@ __device__ void assemble(double *A, double *X, double *B0, double *A0, int n, in
             A[i * LD + j] = v;
             if (COPY) A0[i * MAXM + j] = v;
         }
-        double b[kRhs] = {0.0, 0.0, 0.0, 0.0};
-        if (i < n) {
-            // derivative at the centre (stencil node 0) of the basis function centred at node i
-            const double dx = X[0] - X[i], dy = X[MAXN] - X[MAXN + i], dz = X[2 * MAXN] - X[2 * MAXN + i];
-            const double r = sqrt(dx * dx + dy * dy + dz * dz);
-            if (kind == 0) {
-                // d/dx', d/dy', and (config mode) Delta, Delta^2 of r^k at the centre:
-                // Delta^j r^k = prod_{i<j} (k - 2i)^2 r^(k - 2j) (2D radial Laplacian)
-                const int k = a.phs;
-                const double K = (double)k;
-                b[0] = k * dx * phs_pow(r, k - 2);
-                b[1] = k * dy * phs_pow(r, k - 2);
-                b[2] = K * K * phs_pow(r, k - 2);
-                b[3] = K * K * (K - 2.0) * (K - 2.0) * phs_pow(r, k - 4);
-            } else if (a.ref_mode || a.hvphs == 0) {
-                b[0] = lap_gauss(r, a.eps, a.hv);
-            } else {
-                const double K = (double)a.hvphs;
-                double c = 1.0;
-                for (int jj = 0; jj < a.hv; jj++) c *= (K - 2 * jj) * (K - 2 * jj);
-                b[0] = c * phs_pow(r, a.hvphs - 2 * a.hv);
-            }
-        } else if (kind == 0) {
-            // d/dx' of x' and d/dy' of y' at the origin of the tangent plane; Delta and Delta^2 of
-            // the monomial x^(p-q) y^q there (2 for x^2, y^2; 24, 8, 24 for x^4, x^2 y^2, y^4).
-            // Delta^k (k >= 3) of a monomial of degree < 2k vanishes there.
-            const int t = i - n;
-            int p = 0;
-            while ((p + 1) * (p + 2) / 2 <= t) p++;
-            const int q = t - p * (p + 1) / 2;
-            b[0] = (t == 1) ? 1.0 : 0.0;
-            b[1] = (t == 2) ? 1.0 : 0.0;
-            b[2] = (p == 2 && (q == 0 || q == 2)) ? 2.0 : 0.0;
-            b[3] = p == 4 ? ((q == 0 || q == 4) ? 24.0 : (q == 2 ? 8.0 : 0.0)) : 0.0;
-        }
+        double b[kRhs];
+        rhs_of(i, n, kind, a, X, MAXN, b);
         for (int c = 0; c < nrhs; c++) {
             A[i * LD + m + c] = b[c];
             B0[i * kRhs + c] = b[c];
@@ -350,36 +244,7 @@ __global__ void __launch_bounds__(64) weights_kernel(WArgs a)
         const int M = a.ref_mode ? 21 : (deg + 1) * (deg + 2) / 2;
         const int m1 = n + M;
         const int m2 = gauss_hv ? n + 1 : n + M;
-        double H[9];
-        householder_dev(a.x[node], a.y[node], a.z[node], !a.ref_mode, H);
-        // gather and rotate the stencil (rbffd_DMs.c:159-168)
-        const int *ip = a.idx + (size_t)node * stride;
-        for (int s = lane; s < n; s += 64) {
-            const int q = ip[s];
-            const double px = a.x[q], py = a.y[q], pz = a.z[q];
-            for (int r = 0; r < 3; r++) X[r * MAXN + s] = H[3 * r] * px + H[3 * r + 1] * py + H[3 * r + 2] * pz;
-        }
-        wave_sync();
-        double scale = 1.0;
-        if (!a.ref_mode) {
-            // shift to the centre, scale by the stencil radius
-            const double c0 = X[0], c1 = X[MAXN], c2 = X[2 * MAXN];
-            wave_sync();
-            double rmax = 0.0;
-            for (int s = lane; s < n; s += 64) {
-                const double dx = X[s] - c0, dy = X[MAXN + s] - c1, dz = X[2 * MAXN + s] - c2;
-                X[s] = dx; X[MAXN + s] = dy; X[2 * MAXN + s] = dz;
-                rmax = fmax(rmax, sqrt(dx * dx + dy * dy + dz * dz));
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
-            scale = rmax;
-            wave_sync();
-            for (int s = lane; s < n; s += 64) {
-                X[s] /= scale; X[MAXN + s] /= scale; X[2 * MAXN + s] /= scale;
-            }
-            wave_sync();
-        }
+        const double scale = stencil_frame(a, node, n, X, MAXN, lane);
         int st = 0;
         const size_t row = (size_t)node * stride;
         // gradient system (config mode with closure weights: also Delta and Delta^2)
@@ -448,10 +313,25 @@ int launch_weights(serbf_engine *e, const WArgs &a, int blocks_per_cu)
 // coordinates, or read from a copy of the matrix kept in LDS, when it fits), and the work
 // order (node order, or largest systems first).
 struct WVariant {
-    int copy;      // 1: matrix copy in LDS
+    int rows;      // 1: register-row kernel (systems too large for its classes use the LDS kernel)
+    int copy;      // 1: matrix copy in LDS (LDS kernel)
     int sorted;    // 1: largest systems first
 };
-constexpr WVariant kWVariants[4] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+constexpr int kNVariants = 6;
+constexpr WVariant kWVariants[kNVariants] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1}, {1, 0, 0}, {1, 0, 1}};
+
+// register-row classes: MAXA (columns of A held per row), one wavefront up to 64 rows and two
+// above; a row of 96 + 4 doubles is the largest that leaves the kernel room in the 256
+// addressable vector registers
+constexpr int kNRowClasses = 4;
+constexpr int kRowClasses[kNRowClasses] = {48, 64, 80, 96};
+
+int row_class_of(int m)
+{
+    for (int ci = 0; ci < kNRowClasses; ci++)
+        if (m <= kRowClasses[ci]) return ci;
+    return -1;
+}
 
 template <int MAXM>
 int class_bytes(int copy)
@@ -488,7 +368,8 @@ int launch_class(serbf_engine *e, const WArgs &a, int maxm, int copy)
     return launch_in<112>(e, a, copy);
 }
 
-// system size of every node, the largest one and the order of the nodes by decreasing size
+// gradient-system size of every node, the largest one and the order of the nodes by
+// decreasing size
 int system_sizes(serbf_engine *e, int &maxm, std::vector<int> &by_size, std::vector<int> &m)
 {
     const serbf_config &c = e->cfg;
@@ -544,33 +425,73 @@ int upload_order(serbf_engine *e, const std::vector<int> &order)
 constexpr int kNClasses = 7;
 constexpr int kClasses[kNClasses] = {48, 56, 64, 72, 80, 88, 112};
 
-// One timed run of variant v over the work items `items` (node ids, in the order given).  The
-// items are grouped by LDS class (the smallest class that holds the node's system), and each
-// non-empty group is one persistent launch of its class, largest class first: uneven stencils
-// then run their small systems at the occupancy of a small class.
+// Delta^k system size of node i (Gaussian + constant: n + 1; PHS: the gradient size)
+int hv_size(const serbf_engine *e, int i, int m_grad)
+{
+    const serbf_config &c = e->cfg;
+    const bool gauss = c.mode == SERBF_MODE_REFERENCE || c.hv_phs_order == 0;
+    const int n = e->h_nn.empty() ? e->n : e->h_nn[i];
+    return gauss ? n + 1 : m_grad;
+}
+
+// One timed run of variant v over the work items `items` (node ids, in the order given).
+// LDS kernel: the items are grouped by LDS class (the smallest class that holds the node's
+// systems), and each non-empty group is one persistent launch of its class, largest class
+// first, so uneven stencils run their small systems at the occupancy of a small class.
+// Register-row kernel: the gradient systems grouped by register class, then the Delta^k
+// systems likewise, one persistent launch per non-empty group; nodes with a system larger than
+// the largest register class take the LDS kernel for both.
 int run_variant(serbf_engine *e, int v, const std::vector<int> &m, const std::vector<int> &items, float *ms)
 {
+    const bool hv = e->cfg.hv_order > 0;
+    struct Launch {
+        int off, cnt, engine, cls, kind;   // engine 0 LDS (cls = LDS class index), 1 rows
+    };
+    std::vector<Launch> plan;
     std::vector<int> grouped;
-    grouped.reserve(items.size());
-    int start[kNClasses] = {};
+    grouped.reserve(2 * items.size());
+    std::vector<int> lds_items;
+    if (kWVariants[v].rows) {
+        for (int kind = 0; kind < (hv ? 2 : 1); kind++)
+            for (int ci = kNRowClasses - 1; ci >= 0; ci--) {
+                const int off = (int)grouped.size();
+                for (int it : items) {
+                    const int m1 = m[it], m2 = hv ? hv_size(e, it, m1) : 0;
+                    if (row_class_of(m1) < 0 || row_class_of(m2) < 0) continue;
+                    if (row_class_of(kind == 0 ? m1 : m2) == ci) grouped.push_back(it);
+                }
+                if ((int)grouped.size() > off) plan.push_back({off, (int)grouped.size() - off, 1, ci, kind});
+            }
+        for (int it : items) {
+            const int m1 = m[it], m2 = hv ? hv_size(e, it, m1) : 0;
+            if (row_class_of(m1) < 0 || row_class_of(m2) < 0) lds_items.push_back(it);
+        }
+    } else {
+        lds_items = items;
+    }
     for (int ci = kNClasses - 1; ci >= 0; ci--) {
-        start[ci] = (int)grouped.size();
+        const int off = (int)grouped.size();
         const int lo = ci > 0 ? kClasses[ci - 1] : 0, hi = kClasses[ci];
-        for (int it : items)
+        for (int it : lds_items)
             if (m[it] > lo && m[it] <= hi) grouped.push_back(it);
+        if ((int)grouped.size() > off) plan.push_back({off, (int)grouped.size() - off, 0, ci, 0});
     }
     int rc = upload_order(e, grouped);
     if (rc) return rc;
     SERBF_HIP(hipEventRecord(e->ev0, e->stream));
-    for (int ci = kNClasses - 1; ci >= 0; ci--) {
-        const int off = start[ci], cnt = (ci > 0 ? start[ci - 1] : (int)grouped.size()) - off;
-        if (cnt <= 0) continue;
+    if (!hv) SERBF_HIP(hipMemsetAsync(e->d_WL, 0, (size_t)e->N * e->n * sizeof(double), e->stream));
+    for (const Launch &L : plan) {
         WArgs a = make_args(e);
-        a.order = e->d_worder + off;
-        a.count = cnt;
-        const int copy = kWVariants[v].copy && variant_fits(kClasses[ci], 1) ? 1 : 0;
+        a.order = e->d_worder + L.off;
+        a.count = L.cnt;
         SERBF_HIP(hipMemsetAsync(e->d_scratch, 0, sizeof(int), e->stream));
-        if ((rc = launch_class(e, a, kClasses[ci], copy))) return rc;
+        if (L.engine == 1) {
+            rc = launch_row_class(e, a, L.cls, L.kind);
+        } else {
+            const int copy = kWVariants[v].copy && variant_fits(kClasses[L.cls], 1) ? 1 : 0;
+            rc = launch_class(e, a, kClasses[L.cls], copy);
+        }
+        if (rc) return rc;
     }
     SERBF_HIP(hipEventRecord(e->ev1, e->stream));
     SERBF_HIP(hipEventSynchronize(e->ev1));
@@ -579,6 +500,16 @@ int run_variant(serbf_engine *e, int v, const std::vector<int> &m, const std::ve
 }
 
 }  // namespace
+
+int launch_row_class(serbf_engine *e, const WArgs &a, int ci, int kind)
+{
+    switch (ci) {
+    case 0: return launch_rowlu_48(e, a, kind);
+    case 1: return launch_rowlu_64(e, a, kind);
+    case 2: return launch_rowlu_80(e, a, kind);
+    default: return launch_rowlu_96(e, a, kind);
+    }
+}
 
 int weights_run(serbf_engine *e)
 {
@@ -592,7 +523,7 @@ int weights_run(serbf_engine *e)
     rc = ensure_scratch(e, 64);
     if (rc) return rc;
     int v = e->wvariant;
-    if (v < 0 || v > 3) v = 0;
+    if (v < 0 || v >= kNVariants) v = 4;
     std::vector<int> items(e->N);
     for (int i = 0; i < e->N; i++) items[i] = kWVariants[v].sorted ? by_size[i] : i;
     float ms = 0.f;
@@ -640,7 +571,7 @@ int weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants
     const double systems = (double)sample * (e->cfg.hv_order > 0 ? 2.0 : 1.0);
     int best = 0;
     float best_ms = 1e30f;
-    for (int v = 0; v < 4; v++) {
+    for (int v = 0; v < kNVariants; v++) {
         float ms = -1.f;
         // a copy variant keeps the copy in the classes it fits (it runs without it in the others)
         if (!kWVariants[v].copy || variant_fits(kClasses[0], 1)) {
@@ -657,7 +588,7 @@ int weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants
         }
     }
     if (rc) return rc;
-    if (nvariants) *nvariants = 4;
+    if (nvariants) *nvariants = kNVariants;
     e->wvariant = best;
     return SERBF_OK;
 }

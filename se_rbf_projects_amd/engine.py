@@ -72,6 +72,7 @@ def lib():
             "serbf_get_lap_weights": ([vp, vp, vp], i),
             "serbf_set_stencil_sizes": ([vp, vp, vp], i),
             "serbf_weights_autotune": ([vp, i, vp, vp], i),
+            "serbf_weights_set_variant": ([vp, i], i),
             "serbf_weights_stats": ([vp, vp, vp, vp], i),
             "serbf_closure_create": ([C.POINTER(vp), i], i),
             "serbf_closure_destroy": ([vp], None),
@@ -301,10 +302,14 @@ class Engine:
     def weights_autotune(self, sample=16384):
         """Times the weight-solve schedule variants on a node sample; returns
         [(variant, ms, systems/s)] and keeps the fastest for weights()."""
-        rep = np.zeros(12)
+        rep = np.zeros(3 * 16)
         nv = C.c_int()
         _check(lib().serbf_weights_autotune(self._e, int(sample), _p(rep), C.byref(nv)), "weights_autotune")
         return [(int(rep[3 * k]), float(rep[3 * k + 1]), float(rep[3 * k + 2])) for k in range(nv.value)]
+
+    def weights_variant(self, v):
+        """Pins the schedule variant of weights() (0-5, as weights_autotune numbers them)."""
+        _check(lib().serbf_weights_set_variant(self._e, int(v)), "weights_set_variant")
 
     def weights_stats(self):
         """(kernel ms, systems solved, variant) of the last weights() call."""
