@@ -136,6 +136,16 @@ int serbf_set_lonlat(serbf_engine *e, const double *lambda, const double *phi);
 int serbf_knn(serbf_engine *e, int n);
 int serbf_get_stencils(serbf_engine *e, int *idx, double *D);
 int serbf_set_stencils(serbf_engine *e, int n, const int *idx);
+/* The same kNN search (identical idx and D to serbf_knn + serbf_get_stencils) into caller-held
+ * device memory, allocating nothing: no counterpart in the reference; it is for callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  x, y, z (double[N]) and the outputs idx (int[N*n]) and D (double[N*n],
+ * row-major, nearest first) are device memory on `device`; the search runs on that device's null
+ * stream and returns when done.  Its working arrays are one fixed device array of 61 MiB per GPU,
+ * held until the process ends, which bounds N to 1..1048576 (SERBF_ERR_ARG beyond).  Calls that
+ * overlap on one GPU share that array and race. */
+int serbf_knn_noalloc(int device, int N, const double *x, const double *y, const double *z, int n, int *idx,
+                      double *D);
 
 /* Drops weights and state.  On failure the engine is left unconfigured (configure again before
  * serbf_weights or serbf_set_state). */

@@ -51,6 +51,7 @@ def lib():
             "serbf_knn": ([vp, i], i),
             "serbf_get_stencils": ([vp, vp, vp], i),
             "serbf_set_stencils": ([vp, i, vp], i),
+            "serbf_knn_noalloc": ([i, i, vp, vp, vp, i, vp, vp], i),
             "serbf_configure": ([vp, vp], i),
             "serbf_weights": ([vp], i),
             "serbf_get_weights": ([vp, vp, vp, vp, vp], i),
