@@ -116,7 +116,7 @@ def main():
     ap.add_argument("--closure-every", type=int, default=128, help="steps between closure re-evaluations")
     ap.add_argument("--no-config-c", action="store_true", help="skip the uneven batched weight-solve leg")
     ap.add_argument("--no-config-b", action="store_true", help="skip the FP64 full-period leg at one level lower")
-    ap.add_argument("--kernel-version", type=int, default=5, choices=(2, 5),
+    ap.add_argument("--kernel-version", type=int, default=5, choices=(2, 5, 7),
                     help="stage kernel: 5 matrix-core tiles (default), 2 column kernel")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -262,7 +262,8 @@ def main():
                    "point_update": "one (node, level) advanced one RK4 step (all 4 tracers)",
                    "closure": closure_kind, "closure_every_steps": args.closure_every, "closure_s_max": s_max,
                    "stage_kernel": {2: "column kernel (version 2)",
-                                    5: "matrix-core tiles (version 5)"}[args.kernel_version]
+                                    5: "matrix-core tiles (version 5)",
+                                    7: "FP32 vector tiles (version 7), FP64 matrix-core tiles (version 5)"}[args.kernel_version]
                    + " in both precisions where it applies"},
         "fp64_plain": {"value": value["f64"], "ms_per_step": res["f64"][0] / args.steps,
                        "stage_us": res["f64"][1] * 1e3, "roofline_achieved_GBs": achieved64,

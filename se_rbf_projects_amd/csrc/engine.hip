@@ -592,7 +592,7 @@ int serbf_set_hv_split(serbf_engine *e, int split)
 
 int serbf_set_kernel_version(serbf_engine *e, int version)
 {
-    SERBF_CHECK(e && (version == 2 || version == 5), SERBF_ERR_ARG, "set_kernel_version: 2 or 5");
+    SERBF_CHECK(e && (version == 2 || version == 5 || version == 7), SERBF_ERR_ARG, "set_kernel_version: 2, 5 or 7");
     e->kver = version;
     return SERBF_OK;
 }

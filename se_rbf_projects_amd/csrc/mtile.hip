@@ -312,6 +312,281 @@ mtile2_kernel(StageArgs<T> a, MTilePlan tp, int up)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Version 7 (FP32, 4 tracers): the same tiles and dense weight blocks, with every state access
+// a 16-byte vector.  Wavefront w of a work-group takes tracer w, all 64 levels; lane (c, g) loads
+// levels 4c .. 4c + 3 of union column k0 + 4j + g (one float4) and uses its value j' as the B
+// operand of level block j': block j' computes levels 4n + j' (n = 0..15) of the 16 nodes, so one
+// vector load feeds 4 level blocks x 2 operators = 8 MFMAs and each step of 16 union columns is
+// 4 loads and 32 MFMAs into 8 independent accumulators.  The dense blocks put node t in MFMA row
+// 4 (t % 4) + t / 4, so lane (c, g) ends with nodes g + 4i (i = 0..3) at levels 4c .. 4c + 3 --
+// the same four nodes whose columns it loaded in the first step, where the tile plan lists the
+// tile's own nodes first.  Those own columns give the vertical stencil without another load:
+// the levels of the lanes c - 1 and c + 1 come by DPP row shifts (zero beyond the row, which are
+// the zero ghosts below level 0 and above level 63), and a per-level table of 12 coefficients
+// over levels 4c - 4 .. 4c + 7 (built in LDS at set-up from the vertical stencil) covers the
+// one-sided rows at the ends.  The update's loads and stores are float4 per node.
+constexpr int kVecWin = 12;   // vertical window of a lane: levels 4c - 4 .. 4c + 7
+
+__host__ __device__ inline int mvec_lds_bytes(int up, bool hv)
+{
+    const int wld = up + 1;
+    return (32 + 16 * wld * (hv ? 2 : 1) + kMaxLevels * kVecWin) * 4 + up * 4;
+}
+
+// the value of lane c - 1 (c + 1) of the same 16-lane row, 0 for c = 0 (c = 15)
+__device__ __forceinline__ float row_prev(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float row_next(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x101, 0xF, 0xF, false));
+}
+
+// MFMA row of tile node t: node g + 4i lands in row 4g + i, i.e. in result value i of row group g
+__device__ __forceinline__ int vec_row(int t) { return 4 * (t & 3) + (t >> 2); }
+
+// 4 wavefronts per SIMD (128 registers), 4 work-groups of 4 wavefronts per CU
+template <int STAGE, int HV>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
+{
+    extern __shared__ __align__(16) unsigned char mv_lds[];
+    constexpr int S = 64, NTR = 4, NTH = 256;
+    const int wld = up + 1;
+    float *sv = (float *)mv_lds;                      // [16][2] node velocities
+    float *W = sv + 32;                               // [16][wld] advection weights (row vec_row(t))
+    float *WLt = W + 16 * wld;                        // [16][wld] Delta^k weights
+    float *Ct = WLt + (HV ? 16 * wld : 0);            // [64][kVecWin] vertical coefficients
+    int *uoff = (int *)(Ct + kMaxLevels * kVecWin);   // [up] element offsets of the union columns
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int tile = xcd_remap(blockIdx.x, a.nblk);
+    const int p0 = tile * kMTileNodes;
+    const int U = tp.ucount[tile];
+    const int u16 = (U + 15) & ~15;
+    const int n = a.n;
+    const float *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
+    const int *uni = tp.uni + (size_t)tile * tp.ustride;
+
+    // ---- set-up: weight rows into registers, velocities, union offsets, vertical table ----
+    constexpr int kWE = (kMTileNodes * kMaxStencil + NTH - 1) / NTH;
+    float wxr[kWE], wyr[kWE], wlr[kWE];
+    int kr[kWE];
+    const int ne = (p0 + kMTileNodes <= a.N ? kMTileNodes : a.N - p0) * n;
+#pragma unroll
+    for (int r = 0; r < kWE; r++) {
+        const int e = tid + r * NTH;
+        kr[r] = -1;
+        if (e < ne) {
+            const size_t o = (size_t)p0 * n + e;
+            kr[r] = tp.lidx[o];
+            wxr[r] = a.Wx[o];
+            wyr[r] = a.Wy[o];
+            if (HV) wlr[r] = a.WL[o];
+        }
+    }
+    if (tid < kMTileNodes) {
+        const int p = p0 + tid < a.N ? p0 + tid : a.N - 1;
+        float ux, uy;
+        node_velocity<float>(a, a.basis + (size_t)p * kBasis, ts, ux, uy);
+        sv[2 * tid] = ux;
+        sv[2 * tid + 1] = uy;
+    }
+    for (int k = tid; k < u16; k += NTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
+    for (int e = tid; e < kMaxLevels * kVecWin; e += NTH) {
+        const int l = e / kVecWin, r = e - l * kVecWin;
+        float cv = 0.f;
+        if (l < a.Nv) {
+            const int m = r - (a.vstart[l] - 4 * (l >> 2) + 4);
+            if (m >= 0 && m < a.nv) cv = a.vW[l * kMaxNv + m];
+        }
+        Ct[e] = cv;
+    }
+    for (int e = tid; e < 16 * wld * (HV ? 2 : 1); e += NTH) W[e] = 0.f;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kWE; r++)
+        if (kr[r] >= 0) {
+            const int t = (tid + r * NTH) / n;
+            const int row = vec_row(t);
+            W[row * wld + kr[r]] = sv[2 * t] * wxr[r] + sv[2 * t + 1] * wyr[r];
+            if (HV) WLt[row * wld + kr[r]] = wlr[r];
+        }
+    __syncthreads();
+
+    // ---- D (16 nodes x 64 levels of tracer w) = W (16 x u16) x Q (u16 x 64) ----
+    v4f acc[4], accl[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+        accl[j] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    const float *qw = a.qin + w * S + 4 * c;
+    const float *wa = W + c * wld + g;
+    const float *wl = WLt + c * wld + g;
+    auto step = [&](int k0, const float4 (&b)[4]) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float av = wa[k0 + 4 * j];
+            const float al = HV ? wl[k0 + 4 * j] : 0.f;
+            const float bv[4] = {b[j].x, b[j].y, b[j].z, b[j].w};
+#pragma unroll
+            for (int lb = 0; lb < 4; lb++) {
+                acc[lb] = mfma4(av, bv[lb], acc[lb]);
+                if (HV) accl[lb] = mfma4(al, bv[lb], accl[lb]);
+            }
+        }
+    };
+    // per-level factors: f = alpha r_inv (A q + (u_z / r_inv) Dz q) + c L q, so the vertical term
+    // joins the advection accumulator once the own columns are in
+    const int v0 = 4 * c;
+    const float4 rinv = *(const float4 *)(a.lev + v0);
+    const float4 wprof = *(const float4 *)(a.lev + S + v0);
+    const float wt = a.flow == VEL_DEFORM ? ts[0] : 0.f;
+    // first step: the tile's own columns (union positions 0..15, node 4j + g in load j)
+    float4 own[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) own[j] = *(const float4 *)(qw + uoff[4 * j + g]);
+    step(0, own);
+    {
+        const float sc[4] = {wt * wprof.x / rinv.x, wt * wprof.y / rinv.y, wt * wprof.z / rinv.z,
+                             wt * wprof.w / rinv.w};
+        // vertical stencil of levels 4c .. 4c + 3 of node g + 4i over the window of levels
+        // 4c - 4 .. 4c + 7: the own column and its neighbour lanes' levels
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float ov[4] = {own[i].x, own[i].y, own[i].z, own[i].w};
+            float win[kVecWin];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                win[k] = row_prev(ov[k]);
+                win[4 + k] = ov[k];
+                win[8 + k] = row_next(ov[k]);
+            }
+#pragma unroll
+            for (int lb = 0; lb < 4; lb++) {
+                const float4 *cr = (const float4 *)(Ct + (v0 + lb) * kVecWin);
+                const float4 c0 = cr[0], c1 = cr[1], c2 = cr[2];
+                const float cf[kVecWin] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y, c2.z, c2.w};
+                float zv = 0.f;
+#pragma unroll
+                for (int r = 0; r < kVecWin; r++) zv = fmaf(cf[r], win[r], zv);
+                acc[lb][i] = fmaf(sc[lb], zv, acc[lb][i]);
+            }
+        }
+    }
+    for (int k0 = 16; k0 < u16; k0 += 16) {
+        float4 b[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = *(const float4 *)(qw + uoff[k0 + 4 * j + g]);
+        step(k0, b);
+    }
+
+    // ---- coefficient and RK4 update: nodes g + 4i, levels 4c .. 4c + 3 ----
+    const float rv[4] = {rinv.x, rinv.y, rinv.z, rinv.w};
+    float4 Fv[4], Qv[4];
+    ushort4 Hv[4];
+    float4 Gv[4];
+    size_t mys[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int pr = p0 + g + 4 * i;
+        const int p = pr < a.N ? pr : a.N - 1;
+        mys[i] = ((size_t)p * NTR + w) * S + v0;
+    }
+    // every load of the update is issued before its first store (the stores may alias the
+    // loaded arrays as far as the compiler knows)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (HV && a.hv_mode == 2) Hv[i] = *(const ushort4 *)(a.s16 + mys[i]);
+        else if (HV && a.hv_mode) Gv[i] = *(const float4 *)(a.gamma + mys[i]);
+        Fv[i] = STAGE >= 2 ? *(const float4 *)(a.F + mys[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+        Qv[i] = STAGE >= 1 ? *(const float4 *)(a.qT + mys[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (p0 + g + 4 * i >= a.N) continue;
+        float gv[4] = {a.hv_coef, a.hv_coef, a.hv_coef, a.hv_coef};
+        if (HV && a.hv_mode == 2) {
+            const unsigned short hh[4] = {Hv[i].x, Hv[i].y, Hv[i].z, Hv[i].w};
+#pragma unroll
+            for (int lb = 0; lb < 4; lb++) gv[lb] = a.hv_coef * (float)__builtin_bit_cast(_Float16, hh[lb]);
+        } else if (HV && a.hv_mode) {
+            gv[0] = Gv[i].x; gv[1] = Gv[i].y; gv[2] = Gv[i].z; gv[3] = Gv[i].w;
+        }
+        const float fv[4] = {Fv[i].x, Fv[i].y, Fv[i].z, Fv[i].w};
+        const float qv[4] = {Qv[i].x, Qv[i].y, Qv[i].z, Qv[i].w};
+        float fo[4], qo[4];
+#pragma unroll
+        for (int lb = 0; lb < 4; lb++) {
+            float f = a.alpha * (acc[lb][i] * rv[lb]);
+            if (HV) f += gv[lb] * accl[lb][i];
+            const bool live = v0 + lb < a.Nv;
+            if (!live) f = 0.f;
+            if (STAGE <= 1) {
+                fo[lb] = f;
+                qo[lb] = live ? qv[lb] + a.hdt * f : 0.f;
+            } else if (STAGE == 2 || STAGE == 3) {
+                const float h = STAGE == 2 ? a.hdt : a.dt;
+                fo[lb] = fv[lb] + 2.f * f;
+                qo[lb] = live ? qv[lb] + h * f : 0.f;
+            } else {
+                fo[lb] = 0.f;
+                qo[lb] = live ? qv[lb] + a.dt6 * (fv[lb] + 1.f * f) : 0.f;
+            }
+        }
+        if (STAGE <= 3) *(float4 *)(a.F + mys[i]) = make_float4(fo[0], fo[1], fo[2], fo[3]);
+        if (STAGE >= 1) *(float4 *)(a.qout + mys[i]) = make_float4(qo[0], qo[1], qo[2], qo[3]);
+    }
+}
+
+template <int STAGE, int HV>
+void launch_vec_s(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp, int up, int bytes)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)mvec_kernel<STAGE, HV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((mvec_kernel<STAGE, HV>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
+}
+
+int launch_mvec(serbf_engine *e, int stage, int hv, const StageArgs<float> &a, const MTilePlan &tp)
+{
+    const int up = (e->tile_umax + 15) & ~15;
+    const int bytes = mvec_lds_bytes(up, hv != 0);
+    SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "vector stage kernel: tile union too large");
+    switch (stage * 2 + (hv ? 1 : 0)) {
+    case 0: launch_vec_s<0, 0>(e, a, tp, up, bytes); break;
+    case 1: launch_vec_s<0, 1>(e, a, tp, up, bytes); break;
+    case 2: launch_vec_s<1, 0>(e, a, tp, up, bytes); break;
+    case 3: launch_vec_s<1, 1>(e, a, tp, up, bytes); break;
+    case 4: launch_vec_s<2, 0>(e, a, tp, up, bytes); break;
+    case 5: launch_vec_s<2, 1>(e, a, tp, up, bytes); break;
+    case 6: launch_vec_s<3, 0>(e, a, tp, up, bytes); break;
+    case 7: launch_vec_s<3, 1>(e, a, tp, up, bytes); break;
+    case 8: launch_vec_s<4, 0>(e, a, tp, up, bytes); break;
+    default: launch_vec_s<4, 1>(e, a, tp, up, bytes); break;
+    }
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
+// The vector kernel needs 4 tracers (a wavefront each) and a vertical stencil whose taps for
+// level l lie within levels 4 (l / 4) - 4 .. 4 (l / 4) + 7 (n_v <= 7 centred, one-sided at the ends).
+bool mvec_supported(const serbf_engine *e)
+{
+    if (e->ntr != 4 || e->cfg.Nv > kMaxLevels || e->nv > kMaxNv) return false;
+    for (int l = 0; l < e->cfg.Nv; l++) {
+        const int b = e->h_vstart[l] - 4 * (l >> 2) + 4;
+        if (b < 0 || b + e->nv > kVecWin) return false;
+    }
+    return true;
+}
+
 // tracers per wavefront of the version-5 kernel: 2 (8 wavefronts per work-group).  Measured at
 // 655362 x 60 x 4 with the batched update loads, before the occupancy caps: FP32 1.44 ms per
 // stage (4 tracers per wavefront: 1.44 ms), FP64 3.08 ms (4 tracers: 3.57 ms, one wavefront
@@ -384,6 +659,8 @@ int launch_mtile(serbf_engine *e, int stage, int hv, StageArgs<T> a)
     tp.lidx = e->d_lidx;
     tp.ustride = e->tile_ustride;
     a.nblk = e->ntiles;
+    if constexpr (std::is_same<T, float>::value)
+        if (e->kver == 7 && mvec_supported(e)) return launch_mvec(e, stage, hv, a, tp);
     return launch_mtile2<T>(e, stage, hv, a, tp);
 }
 template int launch_mtile<float>(serbf_engine *, int, int, StageArgs<float>);

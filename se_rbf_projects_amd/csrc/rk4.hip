@@ -535,7 +535,7 @@ void launch2(serbf_engine *e, int stage, int hv, bool xy, StageArgs<T> a)
 template <typename T>
 int launch_stage(serbf_engine *e, int stage, int hv, const StageArgs<T> &a)
 {
-    if (e->kver == 5 && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
+    if ((e->kver == 5 || e->kver == 7) && hv != 2 && mtile_supported(e)) return launch_mtile<T>(e, stage, hv, a);
     const bool xy = a.flow == VEL_TC2 || a.flow == VEL_ARRAY || (a.ref_mode && a.flow == VEL_DEFORM);
     if (e->S == 16) {
         if (e->ntr == 1) launch2<T, 16, 1>(e, stage, hv, xy, a);
@@ -590,7 +590,7 @@ int enqueue_step(serbf_engine *e, StateBuffers<T> &sb, double dt, int step_off)
     // the closure kernel runs at every step and re-evaluates on multiples of closure_every
     // with the FP32 tile kernel the closure hands over its multiplier in FP16 (half the bytes
     // of the per-point coefficient stream that every stage reads)
-    const bool compact = std::is_same<T, float>::value && e->closure && sb.s16 && e->kver == 5 && hv1 != 2 &&
+    const bool compact = std::is_same<T, float>::value && e->closure && sb.s16 && (e->kver == 5 || e->kver == 7) && hv1 != 2 &&
                          mtile_supported(e);
     if (compact) {
         a.hv_mode = 2;
@@ -615,7 +615,9 @@ int run_t(serbf_engine *e, StateBuffers<T> &sb, int precision, int nsteps, doubl
 {
     int rc = ensure_tscal<T>(e, sb, nsteps, dt, t0);
     if (rc) return rc;
-    if (e->kver == 5 && (e->ntr == 4 || e->ntr == 1) && e->S == 64 && e->n <= 128 && (rc = tile_plan(e))) return rc;
+    if ((e->kver == 5 || e->kver == 7) && (e->ntr == 4 || e->ntr == 1) && e->S == 64 && e->n <= 128 &&
+        (rc = tile_plan(e)))
+        return rc;
     // step counter 0 and the closure cadence continued from the steps already run on this
     // state, so that runs of a and b steps refresh the closure as one run of a + b steps
     const int every = e->closure ? e->closure_every : 1;

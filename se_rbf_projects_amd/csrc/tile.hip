@@ -1,11 +1,13 @@
 // Tile plan of the matrix-core stage kernel (mtile.hip), built once per stencil set.
 //
 // A tile is 16 consecutive nodes (nodes are in space-filling order, so a tile is
-// spatially compact).  tile_plan_kernel sorts the tile's 16 x n stencil indices in LDS
-// and stores their union (the distinct neighbour columns, about 126 at n = 55 instead
-// of 880) and, per stencil entry, its 16-bit position in that union.  The stage kernel
-// then reads each union column once per tile and scatters the stencil weights into a
-// dense 16 x union block.
+// spatially compact).  tile_plan_kernel sorts the tile's 16 x n stencil indices together
+// with the tile's own 16 node ids in LDS and stores their union (the distinct columns,
+// about 126 at n = 55 instead of 880) and, per stencil entry, its 16-bit position in that
+// union.  The tile's own nodes come first, in node order (union positions 0..15), the
+// other columns follow in ascending id order: the vector stage kernel takes the tile's own
+// columns from its first step of 16 union columns.  The stage kernels read each union
+// column once per tile and scatter the stencil weights into a dense 16 x union block.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -17,7 +19,7 @@
 namespace serbf {
 
 constexpr int kTileNodes = 16;
-constexpr int kTileKeys = 2048;   // 16 nodes x n <= 128
+constexpr int kTileKeys = 4096;   // 16 nodes x (n + 1) with n <= 128
 
 // ---------------------------------------------------------------- set-up
 __global__ void __launch_bounds__(256) tile_plan_kernel(int N, int n, int umax, const int *idx, int *uni, int *ucount,
@@ -28,14 +30,17 @@ __global__ void __launch_bounds__(256) tile_plan_kernel(int N, int n, int umax, 
     __shared__ int cnt;
     const int t = blockIdx.x, tid = threadIdx.x;
     const int p0 = t * kTileNodes;
+    const int own = N - p0 < kTileNodes ? N - p0 : kTileNodes;   // the tile's own nodes
     const int m = kTileNodes * n;
     int M = 1;
-    while (M < m) M <<= 1;
+    while (M < m + kTileNodes) M <<= 1;
     for (int k = tid; k < M; k += blockDim.x) {
         int v = INT_MAX;
         if (k < m) {
             const int p = p0 + k / n;
             if (p < N) v = idx[(size_t)p * n + k % n];
+        } else if (k < m + own) {
+            v = p0 + (k - m);   // own ids, also where a stencil leaves its centre out
         }
         keys[k] = v;
     }
@@ -68,7 +73,19 @@ __global__ void __launch_bounds__(256) tile_plan_kernel(int N, int n, int umax, 
     }
     __syncthreads();
     const int c = cnt;
-    for (int i = tid; i < c && i < umax; i += blockDim.x) uni[(size_t)t * umax + i] = uniq[i];
+    // the own ids p0 .. p0 + own - 1 are consecutive, so they form one run [o, o + own) of the
+    // sorted union; they move to the front and the columns below them shift up by `own`
+    int o = 0, hi = c;
+    while (o < hi) {
+        const int mid = (o + hi) >> 1;
+        if (uniq[mid] < p0) o = mid + 1;
+        else hi = mid;
+    }
+    auto front = [&](int s) { return s < o ? s + own : (s < o + own ? s - o : s); };
+    for (int i = tid; i < c; i += blockDim.x) {
+        const int j = front(i);
+        if (j < umax) uni[(size_t)t * umax + j] = uniq[i];
+    }
     for (int k = tid; k < m; k += blockDim.x) {
         const int p = p0 + k / n;
         if (p >= N) continue;
@@ -79,7 +96,7 @@ __global__ void __launch_bounds__(256) tile_plan_kernel(int N, int n, int umax, 
             if (uniq[mid] < v) lo = mid + 1;
             else hi = mid;
         }
-        lidx[(size_t)p * n + k % n] = (unsigned short)lo;
+        lidx[(size_t)p * n + k % n] = (unsigned short)front(lo);
     }
 }
 
@@ -87,9 +104,10 @@ __global__ void __launch_bounds__(256) tile_plan_kernel(int N, int n, int umax, 
 int tile_plan(serbf_engine *e)
 {
     if (e->have_plan) return SERBF_OK;
-    SERBF_CHECK(e->n <= kTileKeys / kTileNodes, SERBF_ERR_UNSUPPORTED, "tile plan: n > 128");
+    SERBF_CHECK(e->n <= kTileKeys / kTileNodes - 1 && e->n <= kMaxStencil, SERBF_ERR_UNSUPPORTED,
+                "tile plan: n > 128");
     const int ntiles = (e->N + kTileNodes - 1) / kTileNodes;
-    const int umax = kTileNodes * e->n < 1024 ? kTileNodes * e->n : 1024;
+    const int umax = kTileNodes * (e->n + 1) < 1024 ? kTileNodes * (e->n + 1) : 1024;
     for (void **p : {(void **)&e->d_tuni, (void **)&e->d_tcount, (void **)&e->d_lidx})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     SERBF_HIP(hipMalloc((void **)&e->d_tuni, (size_t)ntiles * umax * sizeof(int)));

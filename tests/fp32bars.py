@@ -14,6 +14,13 @@ BARS = {
     "config_D_shape_v5": (1e-6, 8.43e-8),
     "config_D_shape_v2": (1e-6, 6.67e-8),
     "config_D_shape_v5_no_hv": (6e-7, 5.27e-8),
+    "config_D_shape_v7": (1e-6, 9.30e-08),
+    "config_D_shape_v7_vs_v5": (1e-6, 8.46e-08),
+    "config_D_shape_v7_no_hv": (6e-7, 5.94e-08),
+    "config_D_closure_fp16_v7": (1e-6, 9.23e-08),
+    "levels64_v7_ntr4": (1e-4, 1.15e-05),
+    "levels64_v7_ntr4_vs_column": (2e-4, 2.05e-05),
+    "uneven_vector_tile_vs_column_f32": (1.2e-6, 1.34e-07),
     "config_D_closure_fp16": (1e-6, 8.38e-8),
     "config_A_100_steps": (1.2e-5, 1.09e-6),
     "four_tracers_642x16": (1.2e-6, 1.07e-7),

@@ -93,6 +93,21 @@ def test_config_D_shape_fp32_matches_oracle(deform60):
     assert np.max(np.abs(q[:, 3] - 1.0)) < 1e-5
 
 
+def test_config_D_shape_fp32_vector_kernel_matches_oracle(deform60):
+    """Kernel version 7 (FP32 vector tiles: a wavefront per tracer, float4 state accesses, the
+    vertical stencil from the own columns by row shifts) at the headline's exact shape."""
+    d = deform60
+    q = _run(d["eng"], d["q0"], F32, 7, STEPS, d["dt"])
+    q5 = _run(d["eng"], d["q0"], F32, 5, STEPS, d["dt"])
+    fp32bars.check("config_D_shape_v7", rel_l2(q, d["oq"]))
+    fp32bars.check("config_D_shape_v7_vs_v5", rel_l2(q, q5))
+    assert not np.array_equal(q, q5)                     # the vector kernel ran
+    assert np.max(np.abs(q[:, 3] - 1.0)) < 1e-5
+    # FP64 under version 7 takes the version-5 kernel
+    q64 = _run(d["eng"], d["q0"], F64, 7, STEPS, d["dt"])
+    assert rel_l2(q64, d["oq"]) < 1e-11
+
+
 def test_config_D_shape_without_hyperviscosity(deform60):
     d = deform60
     eng, cfg = d["eng"], d["cfg"]
@@ -103,6 +118,7 @@ def test_config_D_shape_without_hyperviscosity(deform60):
     try:
         q64 = _run(eng, d["q0"], F64, 5, 4, d["dt"])
         q32 = _run(eng, d["q0"], F32, 5, 4, d["dt"])
+        q32v = _run(eng, d["q0"], F32, 7, 4, d["dt"])
         idx, _ = eng.stencils()
         Dx, Dy, L, _ = eng.get_weights()
     finally:
@@ -112,6 +128,7 @@ def test_config_D_shape_without_hyperviscosity(deform60):
     oq = O.Problem(_no_hv(cfg), d["xyz"], idx, Dx, Dy, L).rk4(d["q0"], 4, d["dt"])
     assert rel_l2(q64, oq) < 1e-11
     fp32bars.check("config_D_shape_v5_no_hv", rel_l2(q32, oq))
+    fp32bars.check("config_D_shape_v7_no_hv", rel_l2(q32v, oq))
 
 
 def _no_hv(cfg):
@@ -123,7 +140,8 @@ def _no_hv(cfg):
     return c
 
 
-def test_config_D_closure_fp16_multiplier_matches_oracle(deform60):
+@pytest.mark.parametrize("kver", [5, 7])
+def test_config_D_closure_fp16_multiplier_matches_oracle(deform60, kver):
     """The headline leg's path: the closure's multiplier s handed to the FP32 tile kernel in
     FP16, which forms c = hv_coef * s in every stage.  Against the oracle's RK4 with the same
     per-point coefficient (the FP32 evaluation, rounded through FP16 as the kernel sees it),
@@ -132,7 +150,7 @@ def test_config_D_closure_fp16_multiplier_matches_oracle(deform60):
     eng, cfg, q0, dt = d["eng"], d["cfg"], d["q0"], d["dt"]
     cl = E.Closure(0)
     cl.set_params(CL.init_params(4, out_scale=0.5), np.concatenate([np.zeros(16), np.ones(16)]).astype(np.float32))
-    eng.set_kernel_version(5)
+    eng.set_kernel_version(kver)
     try:
         eng.set_state(q0)
         eng.attach_closure(cl, every=16, s_max=16.0)
@@ -151,6 +169,7 @@ def test_config_D_closure_fp16_multiplier_matches_oracle(deform60):
         q64 = eng.state(F64)
     finally:
         eng.attach_closure(None)
+        eng.set_kernel_version(5)
         cl.close()
     P = d["P"]
     qo = q0.copy()
@@ -161,7 +180,7 @@ def test_config_D_closure_fp16_multiplier_matches_oracle(deform60):
         f3 = P.rhs(qo + dt / 2 * f2, t + dt / 2, gamma)
         f4 = P.rhs(qo + dt * f3, t + dt, gamma)
         qo = qo + dt / 6 * (((f1 + 2 * f2) + 2 * f3) + f4)
-    fp32bars.check("config_D_closure_fp16", rel_l2(q32, qo))
+    fp32bars.check("config_D_closure_fp16" + ("" if kver == 5 else f"_v{kver}"), rel_l2(q32, qo))
     # FP64 with its own (FP64) coefficient against the oracle with that coefficient
     qo = q0.copy()
     for s in range(STEPS):

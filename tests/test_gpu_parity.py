@@ -221,10 +221,11 @@ def test_rk4_four_tracers(eng, golden):
     fp32bars.check("four_tracers_642x16", rel_l2(eng.state(F32), oq))
 
 
-@pytest.mark.parametrize("kver,ntr", [(2, 4), (5, 4), (2, 1), (5, 1)])
+@pytest.mark.parametrize("kver,ntr", [(2, 4), (5, 4), (7, 4), (2, 1), (5, 1)])
 def test_stage_kernels_on_64_levels_match_oracle(eng, golden, kver, ntr):
     """4 tracers (or 1) x 60 levels (padded to 64), deformational flow with Delta^4: the column
-    kernel (2) and the matrix-core tile kernel (5, the default) against the oracle's RK4
+    kernel (2), the matrix-core tile kernel (5) and the FP32 vector tile kernel (7; FP64 runs
+    version 5 there) against the oracle's RK4
     on the oracle's stencils and weights.  2562 nodes leave a partial last 16-node tile."""
     xyz = golden.icos(2562)[:, :3]
     xyz = xyz[E.spatial_order(xyz)]   # compact 16-node tiles: unions of at most 126 columns

@@ -174,7 +174,7 @@ def test_time_loop_on_padded_rows(var):
 def test_tile_kernel_on_uneven_stencils(var):
     """The padded rows of uneven stencils (n = 84 stride, entries past n_i with zero weight) on
     4 tracers x 60 levels: the matrix-core tile kernel (version 5; tile unions of up to 266
-    columns here) against the column kernel, FP64 to rounding, FP32 within its budget and not
+    columns here) and the FP32 vector tile kernel (version 7) against the column kernel, FP64 to rounding, FP32 within its budget and not
     bit-identical (the tile path ran).  Advection only, as in the time-loop test above."""
     eng = var["eng"]
     Dx, Dy, L, _ = var["W"]
@@ -190,7 +190,7 @@ def test_tile_kernel_on_uneven_stencils(var):
     dt = courant_dt(cfg, N)
     out = {}
     try:
-        for kv in (2, 5):
+        for kv in (2, 5, 7):
             eng.set_kernel_version(kv)
             for prec in (F64, F32):
                 eng.set_state(q0)
@@ -207,4 +207,8 @@ def test_tile_kernel_on_uneven_stencils(var):
     assert not np.array_equal(a, b)
     # FP32 sums taken in another order differ by rounding
     fp32bars.check("uneven_tile_vs_column_f32", d)
+    # the FP32 vector tile kernel (version 7) on the same padded rows
+    a = out[(7, F32)]
+    assert not np.array_equal(a, out[(5, F32)])
+    fp32bars.check("uneven_vector_tile_vs_column_f32", np.linalg.norm(a - b) / np.linalg.norm(b))
 
