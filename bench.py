@@ -14,7 +14,8 @@ Timed legs on the same inputs:
   f32+closure  FP32 state and SpMV, hyperviscosity in every stage with the learned
                closure's per-point coefficients (re-evaluated every 128 steps)  -> value
   f32          the same with the constant coefficient (its stage kernel gives the roofline)
-Both precisions use the matrix-core tile stage kernel (kernel version 5).
+Both precisions use 16-node tiles on the matrix cores: FP32 the vector tile kernel (version 7),
+FP64 the matrix-core tile kernel (version 5).
 After the timed legs, the f64 and f32+closure legs run over the full 12-day period of the
 reversing flow (about 7000 steps each, untimed for `value`) and the line states their error
 against the exact solution q(T) = q(0) (--no-period skips this; --period-sweep adds FP32 at
@@ -116,8 +117,9 @@ def main():
     ap.add_argument("--closure-every", type=int, default=128, help="steps between closure re-evaluations")
     ap.add_argument("--no-config-c", action="store_true", help="skip the uneven batched weight-solve leg")
     ap.add_argument("--no-config-b", action="store_true", help="skip the FP64 full-period leg at one level lower")
-    ap.add_argument("--kernel-version", type=int, default=5, choices=(2, 5, 7),
-                    help="stage kernel: 5 matrix-core tiles (default), 2 column kernel")
+    ap.add_argument("--kernel-version", type=int, default=7, choices=(2, 5, 7),
+                    help="stage kernel: 7 FP32 vector tiles + FP64 matrix-core tiles (default), 5 matrix-core "
+                         "tiles in both precisions, 2 column kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not part of "
@@ -238,7 +240,7 @@ def main():
 
     kern = "fp32 step (Delta^4 in every stage)"
     # the committed counter summary is of the default stage kernel
-    traffic = load_traffic(kern) if args.kernel_version == 5 else None
+    traffic = load_traffic(kern) if args.kernel_version == 7 else None
     idx, _ = eng.stencils()
     dense_fl, useful_fl, umean = tile_flops(idx, 64, ntr)
     mfma_tf = dense_fl / (res["f32"][1] / 1e3) / 1e12
@@ -264,7 +266,7 @@ def main():
                    "stage_kernel": {2: "column kernel (version 2)",
                                     5: "matrix-core tiles (version 5)",
                                     7: "FP32 vector tiles (version 7), FP64 matrix-core tiles (version 5)"}[args.kernel_version]
-                   + " in both precisions where it applies"},
+},
         "fp64_plain": {"value": value["f64"], "ms_per_step": res["f64"][0] / args.steps,
                        "stage_us": res["f64"][1] * 1e3, "roofline_achieved_GBs": achieved64,
                        "roofline_frac": achieved64 / HBM_PEAK_GBS},
@@ -278,8 +280,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "frac_of_tuned_copy": achieved / copy_gbs,
-                     "kernel": ("mtile2_kernel<float, STAGE, 1, 2> (matrix-core tiles)" if args.kernel_version == 5
-                                else f"stage kernel version {args.kernel_version}")
+                     "kernel": {7: "mvec_kernel<STAGE, 1> (FP32 vector tiles, constant coefficient)",
+                                5: "mtile2_kernel<float, STAGE, 1, 4, 2> (matrix-core tiles)",
+                                2: "stage2_kernel (column kernel)"}[args.kernel_version]
                                + ", average of the 4 launches of a step",
                      "bytes_per_launch": b32,
                      "note": "average FP32 stage launch of the f32 leg (HIP events); algorithmic bytes model "
@@ -287,7 +290,7 @@ def main():
                      "mfma_view": {"dense_flops_per_launch": dense_fl, "useful_flops_per_launch": useful_fl,
                                    "mean_tile_union": umean, "achieved_TFLOPs": mfma_tf,
                                    "peak_TFLOPs": MFMA_F32_PEAK_TFLOPS, "frac": mfma_tf / MFMA_F32_PEAK_TFLOPS}
-                     if args.kernel_version == 5 else None},
+                     if args.kernel_version in (5, 7) else None},
         "cpu_baseline": cpu,
         "config_c_weights": weights_c,
         "config_b_fp64_period": period_b,

@@ -199,11 +199,11 @@ int serbf_set_hv_split(serbf_engine *e, int split);
 /* Per-point hyperviscosity coefficients c[N*ntracers*Nv] (1/s) replacing hv_coef, e.g.
  * from the learned closure; NULL returns to the constant hv_coef. */
 int serbf_set_gamma(serbf_engine *e, const double *gamma);
-/* Stage kernel variant: 5 (default) matrix-core tiles with the state read directly as the
- * matrix operand (FP32 and FP64, 1 or 4 tracers of 64 levels, config-mode solid-body or
- * deformational flow, hyperviscosity in every stage; other cases use 2); 7 as 5, with FP32
- * at 4 tracers on the vector tile kernel (one wavefront per tracer, 16-byte state accesses,
- * vertical stencils of at most 7 levels); 2 column-vector form (every shape). */
+/* Stage kernel variant: 7 (default) as 5, with FP32 at 4 tracers on the vector tile kernel
+ * (one wavefront per tracer, 16-byte state accesses, vertical stencils of at most 7 levels);
+ * 5 matrix-core tiles with the state read directly as the matrix operand (FP32 and FP64, 1 or
+ * 4 tracers of 64 levels, config-mode solid-body or deformational flow, hyperviscosity in
+ * every stage; other cases use 2); 2 column-vector form (every shape). */
 int serbf_set_kernel_version(serbf_engine *e, int version);
 /* Neighbour columns each wavefront loads before consuming them (4, 8 or 16): the
  * memory-level parallelism / register trade-off of the stage kernel. */

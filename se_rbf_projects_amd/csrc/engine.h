@@ -137,8 +137,8 @@ struct serbf_engine {
     int hv_mode = 0;
     // hyperviscosity once per step (1) or in every stage (0)
     int hv_split = 0;
-    // stage kernel: 3 LDS-tiled, 2 column-vector form, 1 lane-per-level form
-    int kver = 5;
+    // stage kernel: 7 vector tiles for FP32 (matrix-core tiles otherwise), 5 matrix-core tiles, 2 column form
+    int kver = 7;
     // neighbour loads issued together per wavefront in the stage kernel (4, 8, 16)
     int unroll = 8;
     serbf::GraphCache graph;

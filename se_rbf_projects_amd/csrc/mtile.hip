@@ -344,14 +344,31 @@ __device__ __forceinline__ float row_next(float x)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x101, 0xF, 0xF, false));
 }
 
+// 16-byte accesses at a byte offset below 4 GiB from a kernel-argument base: the scalar base plus
+// a 32-bit offset register (global saddr form) instead of a 64-bit address per lane
+template <typename V, typename P>
+__device__ __forceinline__ V ld_at(const P *base, unsigned byte_off)
+{
+    return *(const V *)((const char *)base + byte_off);
+}
+template <typename V, typename P>
+__device__ __forceinline__ void st_at(P *base, unsigned byte_off, const V &v)
+{
+    *(V *)((char *)base + byte_off) = v;
+}
+
 // MFMA row of tile node t: node g + 4i lands in row 4g + i, i.e. in result value i of row group g
 __device__ __forceinline__ int vec_row(int t) { return 4 * (t & 3) + (t >> 2); }
 
-// 4 wavefronts per SIMD (128 registers), 4 work-groups of 4 wavefronts per CU
-template <int STAGE, int HV>
+// 4 wavefronts per SIMD (128 registers), 4 work-groups of 4 wavefronts per CU.  HVM is the
+// hyperviscosity coefficient's source: 0 none, 1 the constant hv_coef, 2 hv_coef times the
+// per-point FP16 multiplier (the closure), 3 the per-point FP32 coefficient; one source per
+// instantiation keeps the update's registers to what that source needs.
+template <int STAGE, int HVM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
 {
+    constexpr bool HV = HVM != 0;
     extern __shared__ __align__(16) unsigned char mv_lds[];
     constexpr int S = 64, NTR = 4, NTH = 256;
     const int wld = up + 1;
@@ -369,6 +386,16 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
     const int n = a.n;
     const float *ts = a.tscal + ((size_t)(*a.base + a.step_off) * 4 + (STAGE == 0 ? 0 : STAGE - 1)) * kTimeScalars;
     const int *uni = tp.uni + (size_t)tile * tp.ustride;
+
+    // the tile's own columns (union positions 0..15) do not need the union list: their loads
+    // go out first and land during the set-up
+    const unsigned lo4 = (unsigned)(w * S + 4 * c) * 4u;   // byte offset of the lane's levels in a node
+    float4 own[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int p = p0 + 4 * j + g < a.N ? p0 + 4 * j + g : a.N - 1;
+        own[j] = ld_at<float4>(a.qin, (unsigned)p * (NTR * S * 4u) + lo4);
+    }
 
     // ---- set-up: weight rows into registers, velocities, union offsets, vertical table ----
     constexpr int kWE = (kMTileNodes * kMaxStencil + NTH - 1) / NTH;
@@ -394,7 +421,7 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
         sv[2 * tid] = ux;
         sv[2 * tid + 1] = uy;
     }
-    for (int k = tid; k < u16; k += NTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S);
+    for (int k = tid; k < u16; k += NTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S * 4);   // bytes
     for (int e = tid; e < kMaxLevels * kVecWin; e += NTH) {
         const int l = e / kVecWin, r = e - l * kVecWin;
         float cv = 0.f;
@@ -406,10 +433,12 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
     }
     for (int e = tid; e < 16 * wld * (HV ? 2 : 1); e += NTH) W[e] = 0.f;
     __syncthreads();
+    const float rn = 1.f / (float)n;
 #pragma unroll
     for (int r = 0; r < kWE; r++)
         if (kr[r] >= 0) {
-            const int t = (tid + r * NTH) / n;
+            // (e + 1/2) / n is at least 1/(2n) away from an integer, far above the rounding
+            const int t = (int)(((float)(tid + r * NTH) + 0.5f) * rn);
             const int row = vec_row(t);
             W[row * wld + kr[r]] = sv[2 * t] * wxr[r] + sv[2 * t + 1] * wyr[r];
             if (HV) WLt[row * wld + kr[r]] = wlr[r];
@@ -417,15 +446,9 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
     __syncthreads();
 
     // ---- D (16 nodes x 64 levels of tracer w) = W (16 x u16) x Q (u16 x 64) ----
-    v4f acc[4], accl[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
-        accl[j] = v4f{0.f, 0.f, 0.f, 0.f};
-    }
-    const float *qw = a.qin + w * S + 4 * c;
     const float *wa = W + c * wld + g;
     const float *wl = WLt + c * wld + g;
+    v4f acc[4], accl[4];
     auto step = [&](int k0, const float4 (&b)[4]) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -439,22 +462,27 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
             }
         }
     };
-    // per-level factors: f = alpha r_inv (A q + (u_z / r_inv) Dz q) + c L q, so the vertical term
-    // joins the advection accumulator once the own columns are in
-    const int v0 = 4 * c;
-    const float4 rinv = *(const float4 *)(a.lev + v0);
-    const float4 wprof = *(const float4 *)(a.lev + S + v0);
-    const float wt = a.flow == VEL_DEFORM ? ts[0] : 0.f;
-    // first step: the tile's own columns (union positions 0..15, node 4j + g in load j)
-    float4 own[4];
+    // The B loads run one step ahead of the MFMAs: the four loads of step k0 + 16 are issued
+    // before the 32 MFMAs of step k0, and a scheduling barrier keeps the compiler from sinking
+    // them (left to itself it issues two loads, waits for them, and so on: two memory round
+    // trips per step).
+    auto load_step = [&](int k0, float4 (&b)[4]) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) own[j] = *(const float4 *)(qw + uoff[4 * j + g]);
-    step(0, own);
+        for (int j = 0; j < 4; j++) b[j] = ld_at<float4>(a.qin, (unsigned)uoff[k0 + 4 * j + g] + lo4);
+    };
+    // first step: the tile's own columns (union positions 0..15, node 4j + g in load j)
+    float4 bn[4];
+    if (u16 > 16) load_step(16, bn);
+    // The vertical term starts the advection accumulator, f = alpha r_inv (A q + (u_z / r_inv)
+    // Dz q) + c L q: the stencil of levels 4c .. 4c + 3 of node g + 4i over the window of levels
+    // 4c - 4 .. 4c + 7, the own column and its neighbour lanes' levels.
+    const int v0 = 4 * c;
     {
+        const float4 rinv = *(const float4 *)(a.lev + v0);
+        const float4 wprof = *(const float4 *)(a.lev + S + v0);
+        const float wt = a.flow == VEL_DEFORM ? ts[0] : 0.f;
         const float sc[4] = {wt * wprof.x / rinv.x, wt * wprof.y / rinv.y, wt * wprof.z / rinv.z,
                              wt * wprof.w / rinv.w};
-        // vertical stencil of levels 4c .. 4c + 3 of node g + 4i over the window of levels
-        // 4c - 4 .. 4c + 7: the own column and its neighbour lanes' levels
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const float ov[4] = {own[i].x, own[i].y, own[i].z, own[i].w};
@@ -473,47 +501,58 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
                 float zv = 0.f;
 #pragma unroll
                 for (int r = 0; r < kVecWin; r++) zv = fmaf(cf[r], win[r], zv);
-                acc[lb][i] = fmaf(sc[lb], zv, acc[lb][i]);
+                acc[lb][i] = sc[lb] * zv;
             }
         }
-    }
-    for (int k0 = 16; k0 < u16; k0 += 16) {
-        float4 b[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) b[j] = *(const float4 *)(qw + uoff[k0 + 4 * j + g]);
-        step(k0, b);
+        for (int lb = 0; lb < 4; lb++) accl[lb] = v4f{0.f, 0.f, 0.f, 0.f};
     }
-
-    // ---- coefficient and RK4 update: nodes g + 4i, levels 4c .. 4c + 3 ----
-    const float rv[4] = {rinv.x, rinv.y, rinv.z, rinv.w};
-    float4 Fv[4], Qv[4];
-    ushort4 Hv[4];
-    float4 Gv[4];
-    size_t mys[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
+    // the update's loads (F, q_T, coefficient)
+    float4 Fv[4], Qv[4], Gv[HVM == 3 ? 4 : 1];
+    ushort4 Hv[HVM == 2 ? 4 : 1];
+    // element index of the lane's first level of node g + 4i (4-byte values; FP16 multipliers at half)
+    auto myi = [&](int i) -> unsigned {
         const int pr = p0 + g + 4 * i;
         const int p = pr < a.N ? pr : a.N - 1;
-        mys[i] = ((size_t)p * NTR + w) * S + v0;
-    }
-    // every load of the update is issued before its first store (the stores may alias the
-    // loaded arrays as far as the compiler knows)
+        return (unsigned)p * (NTR * S) + (unsigned)(w * S + v0);
+    };
+    auto load_update = [&]() {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        if (HV && a.hv_mode == 2) Hv[i] = *(const ushort4 *)(a.s16 + mys[i]);
-        else if (HV && a.hv_mode) Gv[i] = *(const float4 *)(a.gamma + mys[i]);
-        Fv[i] = STAGE >= 2 ? *(const float4 *)(a.F + mys[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
-        Qv[i] = STAGE >= 1 ? *(const float4 *)(a.qT + mys[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 0; i < 4; i++) {
+            const unsigned m = myi(i);
+            if constexpr (HVM == 2) Hv[i] = ld_at<ushort4>(a.s16, m * 2u);
+            if constexpr (HVM == 3) Gv[i] = ld_at<float4>(a.gamma, m * 4u);
+            Fv[i] = STAGE >= 2 ? ld_at<float4>(a.F, m * 4u) : make_float4(0.f, 0.f, 0.f, 0.f);
+            Qv[i] = STAGE >= 1 ? ld_at<float4>(a.qT, m * 4u) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    __builtin_amdgcn_sched_barrier(0);
+    step(0, own);
+    // two steps per iteration with alternating buffers (no register copies)
+    for (int k0 = 16; k0 < u16; k0 += 32) {
+        float4 bm[4];
+        if (k0 + 16 < u16) load_step(k0 + 16, bm);
+        __builtin_amdgcn_sched_barrier(0);
+        step(k0, bn);
+        if (k0 + 16 >= u16) break;
+        if (k0 + 32 < u16) load_step(k0 + 32, bn);
+        __builtin_amdgcn_sched_barrier(0);
+        step(k0 + 16, bm);
     }
+    load_update();
+
+    // ---- coefficient and RK4 update: nodes g + 4i, levels 4c .. 4c + 3 ----
+    const float4 rinv2 = *(const float4 *)(a.lev + v0);
+    const float rv[4] = {rinv2.x, rinv2.y, rinv2.z, rinv2.w};
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         if (p0 + g + 4 * i >= a.N) continue;
         float gv[4] = {a.hv_coef, a.hv_coef, a.hv_coef, a.hv_coef};
-        if (HV && a.hv_mode == 2) {
+        if constexpr (HVM == 2) {
             const unsigned short hh[4] = {Hv[i].x, Hv[i].y, Hv[i].z, Hv[i].w};
 #pragma unroll
             for (int lb = 0; lb < 4; lb++) gv[lb] = a.hv_coef * (float)__builtin_bit_cast(_Float16, hh[lb]);
-        } else if (HV && a.hv_mode) {
+        } else if constexpr (HVM == 3) {
             gv[0] = Gv[i].x; gv[1] = Gv[i].y; gv[2] = Gv[i].z; gv[3] = Gv[i].w;
         }
         const float fv[4] = {Fv[i].x, Fv[i].y, Fv[i].z, Fv[i].w};
@@ -537,21 +576,33 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
                 qo[lb] = live ? qv[lb] + a.dt6 * (fv[lb] + 1.f * f) : 0.f;
             }
         }
-        if (STAGE <= 3) *(float4 *)(a.F + mys[i]) = make_float4(fo[0], fo[1], fo[2], fo[3]);
-        if (STAGE >= 1) *(float4 *)(a.qout + mys[i]) = make_float4(qo[0], qo[1], qo[2], qo[3]);
+        const unsigned m = myi(i) * 4u;
+        if (STAGE <= 3) st_at<float4>(a.F, m, make_float4(fo[0], fo[1], fo[2], fo[3]));
+        if (STAGE >= 1) st_at<float4>(a.qout, m, make_float4(qo[0], qo[1], qo[2], qo[3]));
     }
 }
 
-template <int STAGE, int HV>
+template <int STAGE, int HVM>
 void launch_vec_s(serbf_engine *e, const StageArgs<float> &a, const MTilePlan &tp, int up, int bytes)
 {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)mvec_kernel<STAGE, HV>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void *)mvec_kernel<STAGE, HVM>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((mvec_kernel<STAGE, HV>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
+    hipLaunchKernelGGL((mvec_kernel<STAGE, HVM>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, tp, up);
+}
+
+template <int STAGE>
+void launch_vec_m(serbf_engine *e, int hvm, const StageArgs<float> &a, const MTilePlan &tp, int up, int bytes)
+{
+    switch (hvm) {
+    case 0: launch_vec_s<STAGE, 0>(e, a, tp, up, bytes); break;
+    case 1: launch_vec_s<STAGE, 1>(e, a, tp, up, bytes); break;
+    case 2: launch_vec_s<STAGE, 2>(e, a, tp, up, bytes); break;
+    default: launch_vec_s<STAGE, 3>(e, a, tp, up, bytes); break;
+    }
 }
 
 int launch_mvec(serbf_engine *e, int stage, int hv, const StageArgs<float> &a, const MTilePlan &tp)
@@ -559,17 +610,13 @@ int launch_mvec(serbf_engine *e, int stage, int hv, const StageArgs<float> &a, c
     const int up = (e->tile_umax + 15) & ~15;
     const int bytes = mvec_lds_bytes(up, hv != 0);
     SERBF_CHECK(bytes <= 160 * 1024, SERBF_ERR_UNSUPPORTED, "vector stage kernel: tile union too large");
-    switch (stage * 2 + (hv ? 1 : 0)) {
-    case 0: launch_vec_s<0, 0>(e, a, tp, up, bytes); break;
-    case 1: launch_vec_s<0, 1>(e, a, tp, up, bytes); break;
-    case 2: launch_vec_s<1, 0>(e, a, tp, up, bytes); break;
-    case 3: launch_vec_s<1, 1>(e, a, tp, up, bytes); break;
-    case 4: launch_vec_s<2, 0>(e, a, tp, up, bytes); break;
-    case 5: launch_vec_s<2, 1>(e, a, tp, up, bytes); break;
-    case 6: launch_vec_s<3, 0>(e, a, tp, up, bytes); break;
-    case 7: launch_vec_s<3, 1>(e, a, tp, up, bytes); break;
-    case 8: launch_vec_s<4, 0>(e, a, tp, up, bytes); break;
-    default: launch_vec_s<4, 1>(e, a, tp, up, bytes); break;
+    const int hvm = !hv ? 0 : (a.hv_mode == 2 ? 2 : (a.hv_mode ? 3 : 1));
+    switch (stage) {
+    case 0: launch_vec_m<0>(e, hvm, a, tp, up, bytes); break;
+    case 1: launch_vec_m<1>(e, hvm, a, tp, up, bytes); break;
+    case 2: launch_vec_m<2>(e, hvm, a, tp, up, bytes); break;
+    case 3: launch_vec_m<3>(e, hvm, a, tp, up, bytes); break;
+    default: launch_vec_m<4>(e, hvm, a, tp, up, bytes); break;
     }
     SERBF_HIP(hipGetLastError());
     return SERBF_OK;
@@ -579,7 +626,9 @@ int launch_mvec(serbf_engine *e, int stage, int hv, const StageArgs<float> &a, c
 // level l lie within levels 4 (l / 4) - 4 .. 4 (l / 4) + 7 (n_v <= 7 centred, one-sided at the ends).
 bool mvec_supported(const serbf_engine *e)
 {
-    if (e->ntr != 4 || e->cfg.Nv > kMaxLevels || e->nv > kMaxNv) return false;
+    // 4 tracers of 64 levels, state arrays below 4 GiB (32-bit byte offsets)
+    if (e->ntr != 4 || e->cfg.Nv > kMaxLevels || e->nv > kMaxNv || (size_t)e->N * 4 * 64 * 4 >= (1ull << 32))
+        return false;
     for (int l = 0; l < e->cfg.Nv; l++) {
         const int b = e->h_vstart[l] - 4 * (l >> 2) + 4;
         if (b < 0 || b + e->nv > kVecWin) return false;

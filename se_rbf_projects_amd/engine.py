@@ -396,10 +396,11 @@ class Engine:
         _check(lib().serbf_set_gamma(self._e, _p(g)), "set_gamma")
 
     def set_kernel_version(self, v):
-        """Stage kernel: 5 (default) matrix-core tiles, FP32 and FP64, 1 or 4 tracers of 64
-        levels in config mode; 7 the same tiles with FP32 at 4 tracers on the vector tile
-        kernel (a wavefront per tracer, 16-byte state accesses; FP64 and other shapes as 5);
-        2 column kernel (every shape, and the fallback of the tile kernels)."""
+        """Stage kernel: 7 (default) the 16-node tiles with FP32 at 4 tracers on the vector
+        tile kernel (a wavefront per tracer, 16-byte state accesses) and every other tile shape
+        on the matrix-core tile kernel; 5 the matrix-core tile kernel for FP32 and FP64, 1 or 4
+        tracers of 64 levels in config mode; 2 column kernel (every shape, and the fallback of
+        the tile kernels)."""
         _check(lib().serbf_set_kernel_version(self._e, int(v)), "set_kernel_version")
 
     def tile_info(self):

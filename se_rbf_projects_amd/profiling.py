@@ -98,22 +98,25 @@ def traffic(fetch_dir, write_dir, copy_mib):
         if fk is None or wk is None:
             continue
         out[k] = {"fetch_KiB": fk, "write_KiB": wk, "hbm_bytes_per_launch": kf * fk * 1024.0 + kw * wk * 1024.0}
-    # one RK4 step = stage launches 1..4 with Delta^k (STAGE 0 is the right-hand-side evaluation):
-    # mtile2_kernel<T, STAGE, HV, TPW> or stage2_kernel<T, S, NTR, STAGE, HV, U>
+    # one RK4 step = stage launches 1..4 with Delta^k at the constant coefficient (STAGE 0 is the
+    # right-hand-side evaluation): mvec_kernel<STAGE, HVM> (FP32 only, HVM 1), mtile2_kernel<T,
+    # STAGE, HV, TPW> or stage2_kernel<T, S, NTR, STAGE, HV, U>, the first family found per precision
     for prec in ("float", "double"):
-        for fam, ist, ihv in (("mtile2_kernel", 1, 2), ("stage2_kernel", 3, 4)):
+        key = ("fp32" if prec == "float" else "fp64") + " step (Delta^4 in every stage)"
+        for fam, ist, ihv, pre in (("mvec_kernel", 0, 1, ""), ("mtile2_kernel", 1, 2, prec + ","),
+                                   ("stage2_kernel", 3, 4, prec + ",")):
+            if fam == "mvec_kernel" and prec != "float":
+                continue
             st = []
             for k, val in out.items():
-                if not k.startswith(fam + "<" + prec + ","):
+                if not k.startswith(fam + "<" + pre):
                     continue
                 args = [x.strip() for x in k[k.index("<") + 1:k.rindex(">")].split(",")]
                 if len(args) > ihv and args[ihv] == "1" and args[ist] in ("1", "2", "3", "4"):
                     st.append(val)
-            if len(st) == 4:
-                key = ("fp32" if prec == "float" else "fp64") + " step (Delta^4 in every stage)"
-                if key not in out or fam == "mtile2_kernel":
-                    out[key] = {"hbm_bytes_per_launch": sum(x["hbm_bytes_per_launch"] for x in st) / 4,
-                                "kernel": fam + "<" + prec + ", STAGE, ...>"}
+            if len(st) == 4 and key not in out:
+                out[key] = {"hbm_bytes_per_launch": sum(x["hbm_bytes_per_launch"] for x in st) / 4,
+                            "kernel": fam + "<" + (prec + ", " if pre else "") + "STAGE, ...>"}
     return {"calibration": {"copy_bytes_each_way": nbytes, "fetch_factor": kf, "write_factor": kw},
             "kernels": out}
 

@@ -39,7 +39,7 @@ def deform60():
     P = O.Problem(cfg, xyz, idx, Dx, Dy, L)
     oq = P.rk4(q0, STEPS, dt)
     yield dict(eng=eng, cfg=cfg, q0=q0, dt=dt, P=P, oq=oq, xyz=xyz)
-    eng.set_kernel_version(5)
+    eng.set_kernel_version(7)
     eng.close()
 
 
@@ -169,7 +169,7 @@ def test_config_D_closure_fp16_multiplier_matches_oracle(deform60, kver):
         q64 = eng.state(F64)
     finally:
         eng.attach_closure(None)
-        eng.set_kernel_version(5)
+        eng.set_kernel_version(7)
         cl.close()
     P = d["P"]
     qo = q0.copy()

@@ -260,7 +260,7 @@ def test_stage_kernels_on_64_levels_match_oracle(eng, golden, kver, ntr):
         for t in (0.0, 0.37 * cfg.T):
             assert relmax(eng.rhs(t), P.rhs(q0, t)) < 1e-11
     finally:
-        eng.set_kernel_version(5)
+        eng.set_kernel_version(7)
     oq = P.rk4(q0, 12, dt)
     assert rel_l2(q, oq) < 1e-11
     fp32bars.check(f"levels64_v{kver}_ntr{ntr}", rel_l2(q32, oq))
@@ -437,7 +437,7 @@ def test_run_history_with_closure_equals_one_run(eng):
         cl.close()
 
 
-@pytest.mark.parametrize("kver", [2, 5])
+@pytest.mark.parametrize("kver", [2, 5, 7])
 def test_kernel_versions_agree_with_oracle(eng, probA, kver):
     cfg, xyz, idx, Dx, Dy, L = probA
     q0 = O.initial(E.IC_BELL, cfg, xyz)
@@ -448,7 +448,7 @@ def test_kernel_versions_agree_with_oracle(eng, probA, kver):
         eng.run(F64, 16, dt)
         q = eng.state(F64)
     finally:
-        eng.set_kernel_version(5)
+        eng.set_kernel_version(7)
     oq = O.Problem(cfg, xyz, idx, Dx, Dy, L).rk4(q0, 16, dt)
     assert rel_l2(q, oq) < 1e-11
 

@@ -197,7 +197,7 @@ def test_tile_kernel_on_uneven_stencils(var):
                 eng.run(prec, 10, dt)
                 out[(kv, prec)] = eng.state(prec)
     finally:
-        eng.set_kernel_version(5)
+        eng.set_kernel_version(7)
     a, b = out[(5, F64)], out[(2, F64)]
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-12
     a, b = out[(5, F32)], out[(2, F32)]
