@@ -221,11 +221,14 @@ int serbf_stream_copy(serbf_engine *e, size_t bytes, int reps, double *ms_per_co
 int serbf_get_lap_weights(serbf_engine *e, double *L1, double *L2);
 
 /* ---- learned hyperviscosity closure ----
- * A per-point multiplier s = min(softplus(MLP(x)), s_max) of the hyperviscosity coefficient,
- * c = hv_coef * s, with the MLP 10 -> 64 -> 64 -> 1 (tanh hidden layers) evaluated in BF16
- * on the matrix cores with FP32 accumulation.  s = 1 is the constant-coefficient method,
- * which is also the fallback wherever s is not finite.  s_max keeps c inside the stability
- * region of the explicit RK4 step.
+ * A per-point multiplier s = min(softplus(MLP(x)), s_max, b_p) of the hyperviscosity
+ * coefficient, c = hv_coef * s, with the MLP 10 -> 64 -> 64 -> 1 (ReLU hidden layers)
+ * evaluated in BF16 on the matrix cores with FP32 accumulation.  s = 1 is the
+ * constant-coefficient method, which is also the fallback wherever s is not finite.  s_max
+ * and the optional per-node stability bound b_p keep c inside the stability region of the
+ * explicit RK4 step: b_p = kappa G_ref / G_p with G_p = sum_j |L_pj| the row sum of the node's
+ * Delta^k weights (a Gershgorin radius of the hyperviscosity operator, which scales as h^-8)
+ * and G_ref its 99.9th percentile over the nodes.
  *
  * Features of a point (node p, tracer, level v), with h the node's nearest-neighbour chord
  * (unit sphere) and asinh compression a(y) = asinh(8 y):
@@ -251,6 +254,9 @@ int serbf_closure_set_smax(serbf_closure *c, double s_max);
  * column A[n] and target R[n].  The loss of a sample is ((s - 1) A - R)^2, so s = 1 scores
  * mean(R^2).  Samples are used in the given order (shuffle beforehand). */
 int serbf_closure_set_data(serbf_closure *c, long n, const float *feat, const float *A, const float *R);
+/* Per-sample clamps cap[n] of the training loss, s = min(softplus(o), s_max, cap), for the
+ * samples of the last serbf_closure_set_data (same n); cap = NULL removes them. */
+int serbf_closure_set_caps(serbf_closure *c, long n, const float *cap);
 /* Mean loss over samples [first, first + count) and its gradient: grad float[SERBF_CLOSURE_PARAMS + 1]
  * (the gradient, then the loss).  count must be a multiple of 16. */
 int serbf_closure_grad(serbf_closure *c, long first, long count, float *grad);
@@ -272,6 +278,10 @@ int serbf_closure_features(serbf_engine *e, double t, double dt, float *feat, do
  * c = NULL returns to the constant coefficient.  The engine keeps a pointer to c: detach
  * before destroying it. */
 int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_max);
+/* The per-node stability bound of the attached closure's multiplier: kappa > 0 sets
+ * b_p = kappa G_ref / G_p from the current Delta^k weights (see above; recomputed when new
+ * weights are attached), kappa = 0 removes it.  bound (optional, float[N]) receives b_p. */
+int serbf_closure_bound(serbf_engine *e, double kappa, float *bound);
 /* Evaluate the attached closure once on the state of the given precision at time t (step dt)
  * and return the coefficients gamma double[N*ntr*Nv] (1/s). */
 int serbf_closure_gamma(serbf_engine *e, int precision, double t, double dt, double *gamma);

@@ -3,8 +3,8 @@
  *
  * The closure has no counterpart in the reference (the reference applies a constant
  * hyperviscosity coefficient, PY/Solver.ipynb:428-430); this file restates the network
- * (10 -> 64 -> 64 -> 1, tanh hidden layers, softplus output), its loss, the analytic
- * gradient, Adam, and the per-point features, in double precision.  With emulate != 0 the
+ * (10 -> 64 -> 64 -> 1, ReLU hidden layers, softplus output), its loss with the per-sample
+ * clamps, the analytic gradient, Adam, and the per-point features, in double precision.  With emulate != 0 the
  * forward and backward passes round the operands of every matrix product to bfloat16
  * (round to nearest even) at the same points as the matrix-core kernels, and nowhere else.
  *
@@ -39,6 +39,7 @@ float orc_bf16(float x)
 static double rb(double x, int emulate) { return emulate ? (double)orc_bf16((float)x) : x; }
 
 static double softplus(double o) { return o > 20.0 ? o : log1p(exp(o)); }
+static double relu(double z) { return z > 0.0 ? z : 0.0; }
 static double sigmoid(double o) { return 1.0 / (1.0 + exp(-o)); }
 
 /* network input of a sample: (x_f - shift_f) * scale_f, as float (the kernels normalise in FP32) */
@@ -60,13 +61,13 @@ double orc_mlp_forward(const float *P, const float *norm, const float *feat, int
         double z = 0.0;
         for (int f = 0; f < NF; f++)
             z += rb(P[h * NF + f], emulate) * x[f];
-        H1[h] = tanh((double)(float)z + P[OB1 + h]);
+        H1[h] = relu((double)(float)z + P[OB1 + h]);
     }
     for (int n = 0; n < NH; n++) {
         double z = 0.0;
         for (int h = 0; h < NH; h++)
             z += rb(P[OW2 + n * NH + h], emulate) * rb(H1[h], emulate);
-        H2[n] = tanh((double)(float)z + P[OB2 + n]);
+        H2[n] = relu((double)(float)z + P[OB2 + n]);
     }
     double o = P[OB3];
     for (int n = 0; n < NH; n++)
@@ -75,7 +76,7 @@ double orc_mlp_forward(const float *P, const float *norm, const float *feat, int
 }
 
 double orc_mlp_grad(const float *P, const float *norm, long ns, const float *feat, const float *A,
-                    const float *R, int emulate, double s_max, double *grad)
+                    const float *R, const float *cap, int emulate, double s_max, double *grad)
 {
     memset(grad, 0, sizeof(double) * ORC_CLOSURE_PARAMS);
     double loss = 0.0;
@@ -84,16 +85,20 @@ double orc_mlp_grad(const float *P, const float *norm, long ns, const float *fea
         const float *fs = feat + s * 16;
         const double o = orc_mlp_forward(P, norm, fs, emulate, h1, h2);
         inputs(norm, fs, emulate, x);
-        /* s = min(softplus(o), s_max), no gradient through the clamp (s_max <= 0: no clamp) */
+        /* s = min(softplus(o), s_max, cap[s]), no gradient through the clamp (s_max <= 0 and
+         * cap == NULL: no clamp); the clamp is compared in float, as the kernels do */
+        double smx = s_max;
+        if (cap && (smx <= 0.0 || (double)cap[s] < smx))
+            smx = (double)cap[s];
         const double sp = softplus(o);
-        const int clamped = s_max > 0.0 && sp > s_max;
-        const double e = ((clamped ? s_max : sp) - 1.0) * A[s] - R[s];
+        const int clamped = smx > 0.0 && sp > smx;
+        const double e = ((clamped ? smx : sp) - 1.0) * A[s] - R[s];
         loss += e * e;
         const double go = clamped ? 0.0 : 2.0 * e * A[s] * sigmoid(o) / (double)ns;
         grad[OB3] += go;
         for (int n = 0; n < NH; n++) {
             grad[OW3 + n] += go * h2[n];
-            dz2[n] = go * P[OW3 + n] * (1.0 - h2[n] * h2[n]);
+            dz2[n] = h2[n] > 0.0 ? go * P[OW3 + n] : 0.0;
             grad[OB2 + n] += dz2[n];
         }
         for (int n = 0; n < NH; n++)
@@ -103,7 +108,7 @@ double orc_mlp_grad(const float *P, const float *norm, long ns, const float *fea
             double d = 0.0;
             for (int n = 0; n < NH; n++)
                 d += rb(P[OW2 + n * NH + h], emulate) * rb(dz2[n], emulate);
-            dz1[h] = d * (1.0 - h1[h] * h1[h]);
+            dz1[h] = h1[h] > 0.0 ? d : 0.0;
             grad[OB1 + h] += dz1[h];
         }
         for (int h = 0; h < NH; h++)

@@ -124,7 +124,8 @@ float orc_bf16(float x);
 double orc_mlp_forward(const float *params, const float *norm, const float *feat, int emulate,
                        double *h1, double *h2);
 double orc_mlp_grad(const float *params, const float *norm, long ns, const float *feat,
-                    const float *A, const float *R, int emulate, double s_max, double *grad);
+                    const float *A, const float *R, const float *cap, int emulate, double s_max,
+                    double *grad);
 void orc_adam(int np, float *params, double *m, double *v, const double *g, int step, double lr,
               double b1, double b2, double eps);
 void orc_closure_features(const serbf_config *cfg, int N, const double *x, const double *y,

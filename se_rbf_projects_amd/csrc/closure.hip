@@ -1,6 +1,6 @@
-// Learned hyperviscosity closure (gfx950): per-point features, the 10 -> 64 -> 64 -> 1 MLP in
-// BF16 on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulation), its backward pass,
-// and Adam.
+// Learned hyperviscosity closure (gfx950): per-point features, the 10 -> 64 -> 64 -> 1 MLP
+// (ReLU hidden layers) in BF16 on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32
+// accumulation), its backward pass, Adam, and the per-node stability bound of the multiplier.
 //
 // MFMA operand layout (16x16x16, lane l, c = l % 16, g = l / 16):
 //   A (M x K): A[c][4g + j]      B (K x N): B[4g + j][c]      D (M x N): D[4g + i][c]
@@ -9,6 +9,7 @@
 // pass needs three transposes (H1, dZ2, dZ1), staged through the wavefront's LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -26,6 +27,7 @@ struct serbf_closure {
     bool have_params = false;
     long n = 0, cap = 0;
     float *feat = nullptr, *A = nullptr, *R = nullptr;
+    float *scap = nullptr;        // per-sample clamp of the training loss (optional)
     float *partials = nullptr, *grad = nullptr, *losses = nullptr;
     int loss_cap = 0;
     float *out = nullptr;
@@ -73,14 +75,9 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
 }
 __device__ __forceinline__ float softplus_f(float o) { return o > 20.f ? o : log1pf(expf(o)); }
 __device__ __forceinline__ float sigmoid_f(float o) { return 1.f / (1.f + expf(-o)); }
-// tanh(x) = 1 - 2 / (2^(2x / ln 2) + 1) with the hardware exp2 and reciprocal (v_exp_f32,
-// v_rcp_f32): absolute error ~2e-7, far below the BF16 rounding of the next layer's operand;
-// exp2 overflowing to +inf gives 1, underflowing to 0 gives -1
-__device__ __forceinline__ float tanh_fast(float x)
-{
-    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
-    return fmaf(-2.f, __builtin_amdgcn_rcpf(e + 1.f), 1.f);
-}
+// hidden-layer activation: ReLU, one full-rate instruction per value (the tanh it replaces took
+// two quarter-rate transcendentals, which made the hidden layers the evaluation's bottleneck)
+__device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -136,7 +133,7 @@ __device__ __forceinline__ void fwd_tile(const FwdRegs &r, v4s xb, float h1[4][4
     for (int t = 0; t < 4; t++) {
         const v4f z = mfma(r.w1[t], xb, v4f{r.b1[t][0], r.b1[t][1], r.b1[t][2], r.b1[t][3]});
 #pragma unroll
-        for (int i = 0; i < 4; i++) h1[t][i] = tanh_fast(z[i]);
+        for (int i = 0; i < 4; i++) h1[t][i] = relu(z[i]);
     }
     v4f z2[4];
 #pragma unroll
@@ -150,7 +147,7 @@ __device__ __forceinline__ void fwd_tile(const FwdRegs &r, v4s xb, float h1[4][4
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) h2[t][i] = tanh_fast(z2[t][i]);
+        for (int i = 0; i < 4; i++) h2[t][i] = relu(z2[t][i]);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         float s = 0.f;
@@ -176,6 +173,7 @@ struct FeatArgs {
     const int *origin;       // step index of the run's first step within the closure cadence
     const T *WL1, *WL2;
     const float *hloc;
+    const float *sbound;     // per-node stability bound of the multiplier, or null
     int ntr;
     float hbar, umax, R, dh;
 };
@@ -430,6 +428,7 @@ apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef, floa
             float sv = softplus_f(os[t * 64 + lane]);
             if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
             if (sv > s_max) sv = s_max;
+            if (a.sbound && sv > a.sbound[p]) sv = a.sbound[p];
             if (v >= s.Nv) sv = 0.f;
             const size_t o = ((size_t)p * NT + tr0 + t) * S + v;
             if (s16) s16[o] = (_Float16)sv;
@@ -444,8 +443,8 @@ apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef, floa
 
 // one wavefront per strided set of 16-sample tiles; writes its gradient record to partials
 __global__ void __launch_bounds__(256) train_kernel(NetView nv, const float *feat, const float *A, const float *R,
-                                                    long first, int ntiles, float inv_count, float s_max,
-                                                    float *partials)
+                                                    const float *cap, long first, int ntiles, float inv_count,
+                                                    float s_max, float *partials)
 {
     __shared__ alignas(16) unsigned short w2t[kH * kH];
     __shared__ alignas(16) unsigned short h1s[4][kH * 16];
@@ -488,10 +487,11 @@ __global__ void __launch_bounds__(256) train_kernel(NetView nv, const float *fea
         for (int i = 0; i < 4; i++) {
             const long si = s0 + 4 * g + i;
             const float av = A[si], rv = R[si];
-            // s = min(softplus(o), s_max): no gradient through the clamp
+            // s = min(softplus(o), s_max, cap): no gradient through the clamp
+            const float smx = cap ? fminf(s_max, cap[si]) : s_max;
             const float sp = softplus_f(o[i]);
-            const bool clamped = sp > s_max;
-            const float e = ((clamped ? s_max : sp) - 1.f) * av - rv;
+            const bool clamped = sp > smx;
+            const float e = ((clamped ? smx : sp) - 1.f) * av - rv;
             if (c == 0) loss = fmaf(e, e, loss);
             go[i] = clamped ? 0.f : 2.f * e * av * sigmoid_f(o[i]) * inv_count;
         }
@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(256) train_kernel(NetView nv, const float *fea
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 acc = fmaf(go[i], h2[t][i], acc);
-                dz2[t][i] = go[i] * r.w3[t] * (1.f - h2[t][i] * h2[t][i]);
+                dz2[t][i] = h2[t][i] > 0.f ? go[i] * r.w3[t] : 0.f;
                 bsum += dz2[t][i];
             }
             dw3[t] += acc;
@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(256) train_kernel(NetView nv, const float *fea
                            acc);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                dz1[tm][i] = acc[i] * (1.f - h1[tm][i] * h1[tm][i]);
+                dz1[tm][i] = h1[tm][i] > 0.f ? acc[i] : 0.f;
                 db1[tm][i] += dz1[tm][i];
                 dz1s[w][(16 * tm + 4 * g + i) * 16 + c] = f2bf(dz1[tm][i]);   // dZ1[unit][sample]
             }
@@ -681,8 +681,8 @@ int run_grad(serbf_closure *c, long first, long count, bool update, float lr, in
                 "closure: sample range must lie in the set and be a multiple of 16");
     const int ntiles = (int)(count / 16);
     const int blocks = launch_grid_waves(ntiles);
-    hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(256), 0, c->stream, view(c), c->feat, c->A, c->R, first,
-                       ntiles, 1.f / (float)count, (float)c->s_max, c->partials);
+    hipLaunchKernelGGL(train_kernel, dim3(blocks), dim3(256), 0, c->stream, view(c), c->feat, c->A, c->R, c->scap,
+                       first, ntiles, 1.f / (float)count, (float)c->s_max, c->partials);
     SERBF_HIP(hipGetLastError());
     float bc1 = 1.f, bc2 = 1.f;
     if (update) {
@@ -723,6 +723,7 @@ FeatArgs<T> feat_args(serbf_engine *e, StateBuffers<T> &sb, const T *q, double d
     a.WL1 = sb.WL1;
     a.WL2 = sb.WL2;
     a.hloc = e->d_hloc;
+    a.sbound = e->closure_kappa > 0.0 ? e->d_sbound : nullptr;
     a.ntr = e->ntr;
     a.hbar = (float)std::sqrt(8.0 * SERBF_PI / (std::sqrt(3.0) * e->N));
     a.umax = (float)(e->cfg.flow == SERBF_FLOW_DEFORM ? e->cfg.u0 + e->cfg.k_def : e->cfg.u0);
@@ -739,6 +740,48 @@ int ensure_hloc(serbf_engine *e)
     hipLaunchKernelGGL(hloc_kernel, dim3((e->N + 255) / 256), dim3(256), 0, e->stream, e->N, e->n, e->d_x, e->d_y,
                        e->d_z, e->d_idx, e->d_hloc);
     SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
+// row sums G_p = sum_j |L_pj| of the Delta^k weights
+__global__ void rowsum_kernel(int N, int n, const double *WL, double *G)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    double s = 0.0;
+    for (int k = 0; k < n; k++) s += fabs(WL[(size_t)p * n + k]);
+    G[p] = s;
+}
+
+__global__ void bound_kernel(int N, const double *G, double num, float *b)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    b[p] = G[p] > 0.0 ? (float)fmin(num / G[p], 65504.0) : 65504.f;
+}
+
+// b_p = kappa G_ref / G_p with G_ref the 99.9th percentile of the row sums (a node with a larger
+// row sum than that gets a bound below kappa; the largest 0.1% of the rows do not set the scale)
+int compute_bound(serbf_engine *e)
+{
+    SERBF_CHECK(e->have_weights && e->d_WL, SERBF_ERR_STATE, "closure bound: no Delta^k weights");
+    const int N = e->N;
+    if (!e->d_sbound) SERBF_HIP(hipMalloc((void **)&e->d_sbound, (size_t)N * sizeof(float)));
+    DevTemp<double> G;
+    SERBF_HIP(G.alloc(N));
+    hipLaunchKernelGGL(rowsum_kernel, dim3((N + 255) / 256), dim3(256), 0, e->stream, N, e->n, e->d_WL, G.p);
+    SERBF_HIP(hipGetLastError());
+    std::vector<double> h(N);
+    SERBF_HIP(hipMemcpyAsync(h.data(), G.p, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    SERBF_HIP(hipStreamSynchronize(e->stream));
+    const size_t k = (size_t)(0.999 * (N - 1));
+    std::nth_element(h.begin(), h.begin() + k, h.end());
+    const double gref = h[k];
+    SERBF_CHECK(gref > 0.0, SERBF_ERR_STATE, "closure bound: Delta^k weights are zero");
+    hipLaunchKernelGGL(bound_kernel, dim3((N + 255) / 256), dim3(256), 0, e->stream, N, G.p, e->closure_kappa * gref,
+                       e->d_sbound);
+    SERBF_HIP(hipGetLastError());
+    SERBF_HIP(hipStreamSynchronize(e->stream));
     return SERBF_OK;
 }
 
@@ -859,8 +902,8 @@ void serbf_closure_destroy(serbf_closure *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (float *p : {c->params, c->adam_m, c->adam_v, c->norm, c->feat, c->A, c->R, c->partials, c->grad, c->losses,
-                     c->out})
+    for (float *p : {c->params, c->adam_m, c->adam_v, c->norm, c->feat, c->A, c->R, c->scap, c->partials, c->grad,
+                     c->losses, c->out})
         if (p) (void)hipFree(p);
     for (unsigned short *p : {c->w1b, c->w2b, c->w2tb})
         if (p) (void)hipFree(p);
@@ -909,7 +952,7 @@ int serbf_closure_set_data(serbf_closure *c, long n, const float *feat, const fl
         // a failed allocation leaves no samples and no capacity, not a stale size
         c->n = 0;
         c->cap = 0;
-        for (float **p : {&c->feat, &c->A, &c->R})
+        for (float **p : {&c->feat, &c->A, &c->R, &c->scap})
             if (*p) { (void)hipFree(*p); *p = nullptr; }
         SERBF_HIP(hipMalloc((void **)&c->feat, (size_t)n * 16 * sizeof(float)));
         SERBF_HIP(hipMalloc((void **)&c->A, (size_t)n * sizeof(float)));
@@ -921,6 +964,23 @@ int serbf_closure_set_data(serbf_closure *c, long n, const float *feat, const fl
     SERBF_HIP(hipMemcpyAsync(c->R, R, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     SERBF_HIP(hipStreamSynchronize(c->stream));
     c->n = n;
+    if (c->scap) { (void)hipFree(c->scap); c->scap = nullptr; }   // caps belong to the previous samples
+    return SERBF_OK;
+}
+
+int serbf_closure_set_caps(serbf_closure *c, long n, const float *cap)
+{
+    SERBF_CHECK(c, SERBF_ERR_ARG, "closure_set_caps: null");
+    SERBF_HIP(hipSetDevice(c->device));
+    if (!cap) {
+        if (c->scap) { (void)hipFree(c->scap); c->scap = nullptr; }
+        return SERBF_OK;
+    }
+    SERBF_CHECK(n == c->n && n > 0, SERBF_ERR_ARG, "closure_set_caps: one cap per sample of the training set");
+    // set_data drops the caps, so an existing buffer holds exactly c->n values
+    if (!c->scap) SERBF_HIP(hipMalloc((void **)&c->scap, (size_t)n * sizeof(float)));
+    SERBF_HIP(hipMemcpyAsync(c->scap, cap, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SERBF_HIP(hipStreamSynchronize(c->stream));
     return SERBF_OK;
 }
 
@@ -1039,6 +1099,25 @@ int serbf_closure_attach(serbf_engine *e, serbf_closure *c, int every, double s_
     e->closure_every = every;
     e->closure_smax = s_max;
     e->hv_mode = 1;
+    return e->closure_kappa > 0.0 ? compute_bound(e) : SERBF_OK;
+}
+
+int serbf_closure_bound(serbf_engine *e, double kappa, float *bound)
+{
+    SERBF_CHECK(e && kappa >= 0.0 && std::isfinite(kappa), SERBF_ERR_ARG, "closure_bound: kappa >= 0");
+    SERBF_HIP(hipSetDevice(e->device));
+    if (e->graph.exec) { (void)hipGraphExecDestroy(e->graph.exec); e->graph.exec = nullptr; }
+    e->closure_kappa = kappa;
+    if (kappa == 0.0) return SERBF_OK;
+    int rc = compute_bound(e);
+    if (rc) {
+        e->closure_kappa = 0.0;
+        return rc;
+    }
+    if (bound) {
+        SERBF_HIP(hipMemcpyAsync(bound, e->d_sbound, (size_t)e->N * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+        SERBF_HIP(hipStreamSynchronize(e->stream));
+    }
     return SERBF_OK;
 }
 

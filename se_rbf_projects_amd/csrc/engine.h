@@ -155,6 +155,8 @@ struct serbf_engine {
     serbf_closure *closure = nullptr;
     int closure_every = 8;
     double closure_smax = 8.0;
+    double closure_kappa = 0.0;  // per-node stability bound of the multiplier (0: none)
+    float *d_sbound = nullptr;   // [N] kappa G_ref / G_p
     float *d_hloc = nullptr;   // nearest-neighbour chord per node (closure features)
     int cus = 0;               // compute units of the device (0 until queried)
 };

@@ -171,6 +171,9 @@ static void free_nodes(serbf_engine *e)
     clear_sizes(e);
     if (e->d_hloc) (void)hipFree(e->d_hloc);
     e->d_hloc = nullptr;
+    if (e->d_sbound) (void)hipFree(e->d_sbound);
+    e->d_sbound = nullptr;
+    e->closure_kappa = 0.0;
     e->closure = nullptr;
     e->hv_mode = 0;
     e->d_tuni = e->d_tcount = nullptr;

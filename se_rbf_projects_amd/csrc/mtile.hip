@@ -390,11 +390,21 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
     // the tile's own columns (union positions 0..15) do not need the union list: their loads
     // go out first and land during the set-up
     const unsigned lo4 = (unsigned)(w * S + 4 * c) * 4u;   // byte offset of the lane's levels in a node
+    // In the last tile of a node count that is not a multiple of 16 the positions past its own
+    // nodes hold other union columns: those lanes load the column the first step needs (their
+    // nodes are past N, so their vertical stencil is never used).
     float4 own[4];
+    if (p0 + kMTileNodes <= a.N) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int p = p0 + 4 * j + g < a.N ? p0 + 4 * j + g : a.N - 1;
-        own[j] = ld_at<float4>(a.qin, (unsigned)p * (NTR * S * 4u) + lo4);
+        for (int j = 0; j < 4; j++) own[j] = ld_at<float4>(a.qin, (unsigned)(p0 + 4 * j + g) * (NTR * S * 4u) + lo4);
+    } else {
+        const int nown = a.N - p0, U0 = tp.ucount[tile];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = 4 * j + g;
+            const int id = k < nown ? p0 + k : tp.uni[(size_t)tile * tp.ustride + (k < U0 ? k : 0)];
+            own[j] = ld_at<float4>(a.qin, (unsigned)id * (NTR * S * 4u) + lo4);
+        }
     }
 
     // ---- set-up: weight rows into registers, velocities, union offsets, vertical table ----

@@ -81,11 +81,13 @@ def lib():
             "serbf_closure_get_params": ([vp, vp], i),
             "serbf_closure_set_smax": ([vp, d], i),
             "serbf_closure_set_data": ([vp, C.c_long, vp, vp, vp], i),
+            "serbf_closure_set_caps": ([vp, C.c_long, vp], i),
             "serbf_closure_grad": ([vp, C.c_long, C.c_long, vp], i),
             "serbf_closure_train": ([vp, C.c_long, C.c_long, i, i, d, vp], i),
             "serbf_closure_eval": ([vp, C.c_long, C.c_long, vp], i),
             "serbf_closure_features": ([vp, d, d, vp, vp], i),
             "serbf_closure_attach": ([vp, vp, i, d], i),
+            "serbf_closure_bound": ([vp, d, vp], i),
             "serbf_closure_gamma": ([vp, i, d, d, vp], i),
         }
         for name, (args, res) in sig.items():
@@ -198,6 +200,16 @@ class Closure:
                              f"{self._A.shape}, {self._R.shape}")
         _check(lib().serbf_closure_set_data(self._c, n, _p(self._feat), _p(self._A), _p(self._R)), "closure_set_data")
         self.n = n
+
+    def set_caps(self, cap):
+        """Per-sample clamps of the training loss for the current samples (None removes them)."""
+        if cap is None:
+            _check(lib().serbf_closure_set_caps(self._c, 0, None), "closure_set_caps")
+            return
+        self._cap = np.ascontiguousarray(cap, np.float32)
+        if self._cap.shape != (self.n,):
+            raise ValueError(f"closure caps: one per sample ({self.n}) expected, got {self._cap.shape}")
+        _check(lib().serbf_closure_set_caps(self._c, self.n, _p(self._cap)), "closure_set_caps")
 
     def grad(self, first, count):
         g = np.empty(self.NPARAMS + 1, np.float32)
@@ -434,6 +446,16 @@ class Engine:
         self._closure = closure
         _check(lib().serbf_closure_attach(self._e, closure._c if closure is not None else None, int(every),
                                           float(s_max)), "closure_attach")
+
+    def closure_bound(self, kappa):
+        """Per-node stability bound of the attached closure's multiplier, kappa G_ref / G_p from
+        the Delta^k row sums (kappa = 0 removes it); returns the bound per node (or None)."""
+        if kappa == 0:
+            _check(lib().serbf_closure_bound(self._e, 0.0, None), "closure_bound")
+            return None
+        b = np.empty(self.N, np.float32)
+        _check(lib().serbf_closure_bound(self._e, float(kappa), _p(b)), "closure_bound")
+        return b
 
     def closure_gamma(self, precision, t, dt):
         g = np.empty((self.N, self.ntr, self.cfg.Nv)) if self.ntr > 1 else np.empty((self.N, self.cfg.Nv))

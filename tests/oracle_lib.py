@@ -39,7 +39,7 @@ def lib(path=None):
         _lib.orc_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         _lib.orc_mlp_forward.restype = C.c_double
         _lib.orc_mlp_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
-                                      C.c_int, C.c_double, C.c_void_p]
+                                      C.c_void_p, C.c_int, C.c_double, C.c_void_p]
         _lib.orc_mlp_grad.restype = C.c_double
         _lib.orc_adam.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                                   C.c_double, C.c_double, C.c_double]
@@ -227,15 +227,18 @@ def mlp_forward(params, norm, feat, emulate=True):
                                            int(emulate), None, None) for i in range(len(f))])
 
 
-def mlp_grad(params, norm, feat, A, R, emulate=True, s_max=16.0):
+def mlp_grad(params, norm, feat, A, R, emulate=True, s_max=16.0, cap=None):
+    """Gradient and mean loss of ((min(softplus(o), s_max, cap) - 1) A - R)^2 over the samples."""
     p = _c(params, np.float32)
     nm = _c(norm, np.float32)
     f = _c(feat, np.float32)
     A = _c(A, np.float32)
     R = _c(R, np.float32)
+    cp = None if cap is None else _c(cap, np.float32)
     g = np.zeros(NPARAMS)
     loss = lib().orc_mlp_grad(as_ptr(p, C.c_float), as_ptr(nm, C.c_float), len(A), as_ptr(f, C.c_float),
-                              as_ptr(A, C.c_float), as_ptr(R, C.c_float), int(emulate), float(s_max), as_ptr(g))
+                              as_ptr(A, C.c_float), as_ptr(R, C.c_float), None if cp is None else as_ptr(cp, C.c_float),
+                              int(emulate), float(s_max), as_ptr(g))
     return g, loss
 
 

@@ -34,9 +34,35 @@ def test_bf16_round_to_nearest_even_known_answers():
     assert O.bf16(2.0 ** -130) == 2.0 ** -130                    # subnormals keep their top bits
 
 
+def _margin_problem(seed, m=48):
+    """A problem whose hidden pre-activations all stay at least 0.2 away from zero (half the units
+    on, half off): small weights and biases of +-(0.5 .. 1.5), so a central difference of step 2e-3
+    never crosses a ReLU kink."""
+    params, norm, feat, A, R = _problem(seed, m=m)
+    rng = np.random.default_rng(seed + 100)
+    params[:CL.O_B1] *= 0.2
+    params[CL.O_W2:CL.O_B2] *= 0.2
+    sgn = np.where(np.arange(CL.NH) % 2 == 0, 1.0, -1.0)
+    params[CL.O_B1:CL.O_W2] = sgn * rng.uniform(0.5, 1.5, CL.NH)
+    params[CL.O_B2:CL.O_W3] = sgn[::-1] * rng.uniform(0.5, 1.5, CL.NH)
+    return params, norm, feat, A, R
+
+
+def _preactivations(params, norm, feat):
+    p = params.astype(np.float64)
+    x = ((feat[:, :CL.NF] - norm[:CL.NF]) * norm[16:16 + CL.NF]).astype(np.float64)
+    z1 = x @ p[:CL.O_B1].reshape(CL.NH, CL.NF).T + p[CL.O_B1:CL.O_W2]
+    z2 = np.maximum(z1, 0.0) @ p[CL.O_W2:CL.O_B2].reshape(CL.NH, CL.NH).T + p[CL.O_B2:CL.O_W3]
+    return x, z1, z2
+
+
 def test_gradient_matches_central_differences():
-    """The analytic FP64 gradient of the mean loss against central differences of the loss."""
-    params, norm, feat, A, R = _problem(3)
+    """The analytic FP64 gradient of the mean loss against central differences of the loss, on a
+    network whose ReLU units stay on one side of their kink throughout each difference."""
+    params, norm, feat, A, R = _margin_problem(3)
+    x, z1, z2 = _preactivations(params, norm, feat)
+    assert min(np.min(np.abs(z1)), np.min(np.abs(z2))) > 0.2
+    assert 0.3 < np.mean(z1 > 0) < 0.7 and 0.3 < np.mean(z2 > 0) < 0.7
     g, loss = O.mlp_grad(params, norm, feat, A, R, emulate=False)
     rng = np.random.default_rng(0)
     picks = np.concatenate([rng.choice(CL.O_B1, 8, replace=False),                 # W1
@@ -47,12 +73,77 @@ def test_gradient_matches_central_differences():
     scale = np.max(np.abs(g))
     for k in picks:
         hp, hm = params.copy(), params.copy()
-        hp[k] += np.float32(1e-2)
-        hm[k] -= np.float32(1e-2)
+        hp[k] += np.float32(2e-3)
+        hm[k] -= np.float32(2e-3)
         lp = O.mlp_grad(hp, norm, feat, A, R, emulate=False)[1]
         lm = O.mlp_grad(hm, norm, feat, A, R, emulate=False)[1]
         fd = (lp - lm) / (float(hp[k]) - float(hm[k]))
         assert abs(fd - g[k]) < 2e-4 * scale + 1e-3 * abs(g[k]), (k, fd, g[k])
+
+
+def _numpy_forward(params, norm, feat):
+    """The network in FP64 numpy from the parameter layout W1[64][10], b1, W2[64][64], b2, w3, b3,
+    with the oracle's float rounding points (normalised input and each layer's product)."""
+    p = params.astype(np.float64)
+    W1 = p[:CL.O_B1].reshape(CL.NH, CL.NF)
+    b1 = p[CL.O_B1:CL.O_W2]
+    W2 = p[CL.O_W2:CL.O_B2].reshape(CL.NH, CL.NH)
+    b2 = p[CL.O_B2:CL.O_W3]
+    w3 = p[CL.O_W3:CL.O_B3]
+    x = ((feat[:, :CL.NF] - norm[:CL.NF]) * norm[16:16 + CL.NF]).astype(np.float32).astype(np.float64)
+    h1 = np.maximum((x @ W1.T).astype(np.float32).astype(np.float64) + b1, 0.0)
+    h2 = np.maximum((h1 @ W2.T).astype(np.float32).astype(np.float64) + b2, 0.0)
+    return h2 @ w3 + p[CL.O_B3]
+
+
+def test_forward_matches_numpy_relu_network():
+    """The oracle's forward pass (FP64, no BF16 emulation) against the same network written with
+    numpy matrix products: the parameter layout, the ReLU and the output layer."""
+    params, norm, feat, A, R = _problem(4, m=64)
+    o = O.mlp_forward(params, norm, feat, emulate=False)
+    ref = _numpy_forward(params, norm, feat)
+    assert np.max(np.abs(o - ref)) < 1e-6 * max(1.0, np.max(np.abs(ref)))
+    # both hidden layers have active and inactive units on this problem
+    p = params.astype(np.float64)
+    x = ((feat[:, :CL.NF] - norm[:CL.NF]) * norm[16:16 + CL.NF]).astype(np.float64)
+    z1 = x @ p[:CL.O_B1].reshape(CL.NH, CL.NF).T + p[CL.O_B1:CL.O_W2]
+    assert 0.2 < np.mean(z1 > 0) < 0.8
+
+
+def test_dead_first_layer_closed_form():
+    """b1 = -100: every first-layer unit is off (H1 = 0), so the output is b3 + w3 . relu(b2) for every
+    sample and neither W1, b1 nor W2 gets a gradient."""
+    params, norm, feat, A, R = _problem(6)
+    params[CL.O_B1:CL.O_W2] = -100.0
+    o = O.mlp_forward(params, norm, feat, emulate=False)
+    p = params.astype(np.float64)
+    const = p[CL.O_B3] + np.dot(p[CL.O_W3:CL.O_B3], np.maximum(p[CL.O_B2:CL.O_W3], 0.0))
+    assert np.allclose(o, const, rtol=0, atol=1e-12)
+    g, _ = O.mlp_grad(params, norm, feat, A, R, emulate=False, s_max=1e6)
+    assert np.all(g[:CL.O_B2] == 0.0)
+    assert np.any(g[CL.O_B2:] != 0.0)
+
+
+def test_per_sample_caps():
+    """s = min(softplus(o), s_max, cap): samples whose cap is below softplus(o) score
+    ((cap - 1) A - R)^2 and pass no gradient; caps above s_max change nothing."""
+    params, norm, feat, A, R = _problem(10, m=64)
+    params[CL.O_W3:CL.O_B3] = 0.0
+    params[CL.O_B3] = 5.0                                    # softplus(5) = 5.0067 for every sample
+    sp = math.log1p(math.exp(5.0))
+    cap = np.where(np.arange(64) % 2 == 0, 3.0, 100.0).astype(np.float32)
+    g, loss = O.mlp_grad(params, norm, feat, A, R, emulate=False, s_max=16.0, cap=cap)
+    s = np.where(cap < sp, cap.astype(np.float64), sp)
+    assert loss == pytest.approx(np.mean(((s - 1.0) * A - R) ** 2), rel=1e-12)
+    # the output gradient comes from the uncapped (odd) samples only
+    sig = 1.0 / (1.0 + math.exp(-5.0))
+    odd = np.arange(64) % 2 == 1
+    Ad, Rd = A.astype(np.float64), R.astype(np.float64)
+    expect = np.sum(2.0 * ((sp - 1.0) * Ad[odd] - Rd[odd]) * Ad[odd]) * sig / 64
+    assert g[CL.O_B3] == pytest.approx(expect, rel=1e-10)
+    g_hi, l_hi = O.mlp_grad(params, norm, feat, A, R, emulate=False, s_max=16.0, cap=np.full(64, 50.0, np.float32))
+    g_no, l_no = O.mlp_grad(params, norm, feat, A, R, emulate=False, s_max=16.0)
+    assert l_hi == l_no and np.array_equal(g_hi, g_no)
 
 
 def test_constant_method_loss_and_output_gradient_closed_form():

@@ -64,6 +64,33 @@ def test_backward_matches_oracle():
     assert l2 == pytest.approx(lref2, rel=1e-4)
 
 
+def test_backward_with_caps_matches_oracle():
+    """Per-sample clamps of the loss: half the samples capped at half their multiplier, half at
+    twice it (no sample near its cap), against the oracle with the same caps; removing the caps
+    and replacing the samples both return the uncapped loss."""
+    params, norm, feat, A, R = _problem(14, m=1024)
+    o = O.mlp_forward(params, norm, feat, emulate=True)
+    sp = np.log1p(np.exp(np.minimum(o, 30.0)))
+    half = np.arange(1024) % 2 == 0
+    cap = np.where(half, 0.5 * sp, 2.0 * sp).astype(np.float32)
+    cl = E.Closure(0)
+    cl.set_params(params, norm)
+    cl.set_data(feat, A, R)
+    cl.set_caps(cap)
+    g, loss = cl.grad(0, 1024)
+    ref, lref = O.mlp_grad(params, norm, feat, A, R, emulate=True, cap=cap)
+    assert loss == pytest.approx(lref, rel=1e-4)
+    assert np.linalg.norm(g - ref) / np.linalg.norm(ref) < 3e-3
+    _, lnone = O.mlp_grad(params, norm, feat, A, R, emulate=True)
+    assert abs(lnone - lref) > 1e-3 * lref            # the caps change the loss
+    cl.set_caps(None)
+    assert cl.grad(0, 1024)[1] == pytest.approx(lnone, rel=1e-4)
+    cl.set_caps(cap)
+    cl.set_data(feat, A, R)                           # new samples drop the caps
+    assert cl.grad(0, 1024)[1] == pytest.approx(lnone, rel=1e-4)
+    cl.close()
+
+
 def test_adam_steps_match_oracle():
     params, norm, feat, A, R = _problem(13, m=512)
     cl = E.Closure(0)
@@ -126,14 +153,25 @@ def test_features_match_oracle(small):
     assert np.max(np.abs(lap - lref)) < 1e-10 * np.max(np.abs(lref))
 
 
-def _oracle_gamma(small, params, norm, t, q, s_max=8.0):
+def _oracle_bound(L, kappa):
+    """kappa G_ref / G_p: G_p the Delta^k row sums, G_ref the 99.9th percentile (element
+    floor(0.999 (N - 1)) of the sorted sums)."""
+    G = np.abs(L).sum(axis=1)
+    k = int(0.999 * (len(G) - 1))
+    gref = np.partition(G, k)[k]
+    return np.minimum(kappa * gref / G, 65504.0).astype(np.float32)
+
+
+def _oracle_gamma(small, params, norm, t, q, s_max=8.0, bound=None):
     cfg = small["cfg"]
     feat, _ = O.closure_features(cfg, small["xyz"], small["idx"], *small["W"], small["vstart"], small["vW"], q, t,
                                  small["dt"])
     o = O.mlp_forward(params, norm, feat, emulate=True)
     s = np.log1p(np.exp(np.minimum(o, 30.0)))
-    s = np.where(np.isfinite(s), np.minimum(s, s_max), 1.0)
-    return cfg.hv_coef * s.reshape(q.shape), feat
+    s = np.where(np.isfinite(s), np.minimum(s, s_max), 1.0).reshape(q.shape)
+    if bound is not None:
+        s = np.minimum(s, bound.astype(np.float64).reshape((-1,) + (1,) * (q.ndim - 1)))
+    return cfg.hv_coef * s, feat
 
 
 def test_coefficients_match_oracle_and_fall_back(small):
@@ -165,6 +203,23 @@ def test_coefficients_match_oracle_and_fall_back(small):
     cl.set_params(bad, norm)
     gn = eng.closure_gamma(F64, t, small["dt"])
     assert np.all(gn == cfg.hv_coef)
+    # the per-node stability bound kappa G_ref / G_p caps the multiplier node by node
+    cl.set_params(params, norm)
+    # kappa putting the median bound at the median multiplier: about half the points capped
+    kappa = float(np.median(ref / cfg.hv_coef) / np.median(_oracle_bound(small["W"][2], 1.0)))
+    b = eng.closure_bound(kappa)
+    bref = _oracle_bound(small["W"][2], kappa)
+    assert np.allclose(b, bref, rtol=1e-6)
+    gbd = eng.closure_gamma(F64, t, small["dt"])
+    refb, _ = _oracle_gamma(small, params, norm, t, small["q0"], bound=bref)
+    relb = np.abs(gbd - refb) / abs(cfg.hv_coef)
+    capped = (ref / cfg.hv_coef) > bref[:, None, None] * 1.01
+    print("bound: capped share", capped.mean())
+    assert 0.05 < capped.mean() < 0.95
+    assert np.mean(relb < 2e-3) > 0.999 and relb.max() < 5e-2
+    assert np.all(gbd[capped] == (cfg.hv_coef * bref[:, None, None].astype(np.float64) * np.ones_like(gbd))[capped])
+    eng.closure_bound(0.0)                              # removed: back to the unbounded field
+    assert np.array_equal(eng.closure_gamma(F64, t, small["dt"]), g64)
     eng.attach_closure(None)
     cl.close()
 
@@ -215,12 +270,14 @@ def test_rk4_with_closure_matches_oracle(golden):
 
 
 def test_trained_closure_gains_over_constant_coefficient():
-    """Train on coarse (642 nodes) vs restricted fine (10242 nodes) runs of the deformational
-    flow.  On held-out samples the trained network must lower the M-step error the constant
-    coefficient leaves (normalised to 1), and do at least as well as the best constant multiplier
-    the stability clamp admits."""
-    feat, A, R, meta = CL.make_dataset(fine_level=5, coarse_level=3, Nv=10, n=31, deg=4, nv=5, snapshots=6)
-    params, norm, rep = CL.train(feat, A, R, steps=1500, batch=2048, lr=3e-3, log_every=500)
+    """Train on coarse (642 nodes) vs restricted fine (2562 nodes) runs of the deformational
+    flow, with every sample clamped at its node's stability bound.  On held-out samples the
+    trained network must lower the M-step error the constant coefficient leaves (normalised to
+    1), and do at least as well as the best constant multiplier every clamp admits."""
+    feat, A, R, cap, meta = CL.make_dataset(fine_level=4, coarse_level=3, Nv=10, n=31, deg=4, nv=5, snapshots=6,
+                                            keep=1.0, kappa=CL.KAPPA)
+    assert cap is not None and len(cap) == len(A) and cap.min() > 1.0
+    params, norm, rep = CL.train(feat, A, R, cap=cap, steps=1500, batch=2048, lr=3e-3, log_every=500)
     print({k: v for k, v in rep.items() if k.startswith("loss") or k.startswith("best") or k.startswith("s_")})
     assert rep["loss_const_holdout"] == pytest.approx(1.0, rel=0.2)
     assert rep["loss_trained_holdout"] < 0.995 * rep["loss_const_holdout"]

@@ -108,6 +108,30 @@ def test_config_D_shape_fp32_vector_kernel_matches_oracle(deform60):
     assert rel_l2(q64, d["oq"]) < 1e-11
 
 
+def test_partial_last_tile_with_fields_there(deform60):
+    """10242 nodes leave a last tile of 2 nodes whose union positions 2..15 are other nodes'
+    columns.  A smooth field that is not constant on those nodes from the first step (the
+    configured tracers are zero there at first): one FP32 RK4 step of the vector kernel (7) and the
+    matrix-core kernel (5) against the oracle, at the last tile's nodes and everywhere."""
+    d = deform60
+    eng, xyz, dt = d["eng"], d["xyz"], d["dt"]
+    N = len(xyz)
+    assert N % 16 == 2
+    lev = (np.arange(60) + 0.5) / 60
+    q0 = np.stack([np.sin(3 * xyz[:, 0:1] + t) * np.cos(2 * xyz[:, 1:2]) + 0.1 * xyz[:, 2:3] * lev[None, :]
+                   for t in range(4)], axis=1)
+    oq = d["P"].rk4(q0, 1, dt)
+    last = slice(N - N % 16, N)
+    dq = np.max(np.abs(oq - q0))
+    for kver in (7, 5):
+        q = _run(eng, q0, F32, kver, 1, dt)
+        err_last = np.max(np.abs(q[last] - oq[last])) / dq
+        print(f"kernel {kver}: last tile error / step change {err_last:.2e}, rel l2 {rel_l2(q, oq):.2e}")
+        assert err_last < 1e-3
+        assert rel_l2(q, oq) < 1e-6
+    eng.set_kernel_version(7)
+
+
 def test_config_D_shape_without_hyperviscosity(deform60):
     d = deform60
     eng, cfg = d["eng"], d["cfg"]
