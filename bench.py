@@ -19,7 +19,7 @@ FP64 the matrix-core tile kernel (version 5).
 After the timed legs, the f64 and f32+closure legs run over the full 12-day period of the
 reversing flow (about 7000 steps each, untimed for `value`) and the line states their error
 against the exact solution q(T) = q(0) (--no-period skips this; --period-sweep adds FP32 at
-constant coefficients for comparison with the closure).
+constant coefficients and at the closure's per-node stability bound alone, for comparison).
 Secondary legs (reported beside the headline, not in it): the uneven batched weight solves at
 the same level with a LAPACK residual on sampled nodes (config C), and the plain FP64 method
 over a full period one level lower (163842 nodes x 60, one field of two Gaussian hills;
@@ -111,8 +111,8 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=1024, help="size of the stream-copy bandwidth reference")
     ap.add_argument("--no-period", action="store_true", help="skip the full-period error legs")
     ap.add_argument("--period-sweep", action="store_true",
-                    help="with the period legs, also FP32 at constant coefficients (1, s_max and the multipliers "
-                         "given by --sweep-multipliers)")
+                    help="with the period legs, also FP32 at constant coefficients (1, 28, 30 and the multipliers "
+                         "given by --sweep-multipliers) and at the closure's stability bound alone")
     ap.add_argument("--sweep-multipliers", type=str, default="", help="comma-separated extra constant multipliers")
     ap.add_argument("--closure-every", type=int, default=128, help="steps between closure re-evaluations")
     ap.add_argument("--no-config-c", action="store_true", help="skip the uneven batched weight-solve leg")
@@ -164,14 +164,20 @@ def main():
         params, norm, rep = CL.init_params(0), np.concatenate([np.zeros(16), np.ones(16)]).astype(np.float32), {}
         closure_kind = "untrained (constant multiplier 1; no closure_params.npz)"
     s_max = float(rep.get("s_max", CL.S_MAX))
+    kappa = rep.get("kappa")    # the per-node stability bound the network was trained under
     closure = E.Closure(0)
     closure.set_params(params, norm)
+
+    def attach(use_cl):
+        eng.attach_closure(closure if use_cl else None, every=args.closure_every, s_max=s_max)
+        if use_cl and kappa:
+            eng.closure_bound(kappa)
 
     legs = (("f64", F64, False, False), ("f32+closure", F32, False, True), ("f32", F32, False, False))
     res, final = {}, {}
     for name, prec, split, use_cl in legs:
         eng.set_hv_split(split)
-        eng.attach_closure(closure if use_cl else None, every=args.closure_every, s_max=s_max)
+        attach(use_cl)
         eng.set_state(q0)
         eng.run(prec, args.warmup, dt)
         eng.set_state(q0)
@@ -196,7 +202,7 @@ def main():
                   "norms": "relative l2 / linf over the three transported tracers"}
         for name, prec, split, use_cl in legs[:2]:
             eng.set_hv_split(split)
-            eng.attach_closure(closure if use_cl else None, every=args.closure_every, s_max=s_max)
+            attach(use_cl)
             eng.set_state(q0)
             t = time.time()
             eng.run(prec, nsteps_T, dt)
@@ -207,22 +213,24 @@ def main():
             log(f"period {name}: {period[name]}")
         period["f32+closure_vs_f64_l2"] = period["f32+closure"]["rel_l2"] / period["f64"]["rel_l2"]
         if args.period_sweep:
+            # the closure against the constant coefficient at the largest stable multipliers and
+            # against its stability bound alone (s = kappa G_ref / G_p at every point)
             eng.set_hv_split(False)
             eng.attach_closure(None)
-            mults = [1.0, s_max] + [float(m) for m in args.sweep_multipliers.split(",") if m.strip()]
-            for mlt in mults:
-                cfgm = config_deform(N, Nv=args.nv, n=args.n, deg=4, nv=7)
-                cfgm.ntracers = ntr
-                cfgm.hv_coef = cfg.hv_coef * mlt
-                eng.configure(cfgm)
-                eng.weights()
+            mults = [1.0, 28.0, 30.0] + [float(m) for m in args.sweep_multipliers.split(",") if m.strip()]
+            fields = [(f"f32 constant coefficient x {m:g}", np.full(N, m)) for m in mults]
+            if kappa:
+                b = eng.closure_bound(kappa)
+                eng.closure_bound(0.0)
+                fields.append((f"f32 stability bound alone (kappa {kappa:g})", np.minimum(b.astype(np.float64), s_max)))
+            for key, sfield in fields:
+                eng.set_state(q0)
+                eng.set_gamma(np.repeat(cfg.hv_coef * sfield[:, None], ntr * cfg.Nv, axis=1))
                 eng.set_state(q0)
                 eng.run(F32, nsteps_T, dt)
-                key = f"f32 constant coefficient x {mlt:g}"
                 period[key] = error_vs(eng.state(F32), q0)
+                eng.set_gamma(None)
                 log(f"period {key}: {period[key]}")
-            eng.configure(cfg)
-            eng.weights()
             eng.set_state(q0)
     eng.attach_closure(None)
     eng.set_hv_split(False)
@@ -263,6 +271,7 @@ def main():
                    "nodes": N, "levels": cfg.Nv, "tracers": ntr, "stencil": cfg.n, "dt_s": dt,
                    "point_update": "one (node, level) advanced one RK4 step (all 4 tracers)",
                    "closure": closure_kind, "closure_every_steps": args.closure_every, "closure_s_max": s_max,
+                   "closure_stability_bound_kappa": kappa,
                    "stage_kernel": {2: "column kernel (version 2)",
                                     5: "matrix-core tiles (version 5)",
                                     7: "FP32 vector tiles (version 7), FP64 matrix-core tiles (version 5)"}[args.kernel_version]
