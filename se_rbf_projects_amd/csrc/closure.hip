@@ -12,10 +12,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "engine.h"
 #include "stage_common.h"
+#include "tile_common.h"
 
 struct serbf_closure {
     int device = 0;
@@ -157,6 +159,95 @@ __device__ __forceinline__ void fwd_tile(const FwdRegs &r, v4s xb, float h1[4][4
         for (int m = 1; m < 16; m <<= 1) s += __shfl_xor(s, m, 64);
         o[i] = s + r.b3;
     }
+}
+
+// ---- the forward pass of the evaluation kernels: every layer as W H (M = unit, N = sample), so
+// that a layer's result tile is already the next layer's B operand and the output reduction is two
+// cross-row shuffles; layer 2 on v_mfma_f32_16x16x32_bf16.  Sample c of a tile lives in lanes
+// (c, g), g = 0..3, which all end with its output.
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ v4f mfma32(v8s a, v8s b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0,
+                                                    0);
+}
+
+// Layer 2's K block kk (32 units) in lane row g holds units 32 kk + 4g + i (i = 0..3) and
+// 32 kk + 16 + 4g + i: the layer-1 result tiles 2kk and 2kk + 1 of that lane, packed as they are.
+struct InferRegs {
+    v4s w1[4];        // W1[16t + c][4g .. 4g + 3]
+    v8s w2[2][4];     // W2[16tm + c][units of K block kk in row g]
+    float b1[4][4];   // b1[16t + 4g + i]
+    float b2[4][4];   // b2[16tm + 4g + i]
+    float w3[4][4];   // w3[16tm + 4g + i]
+    float b3;
+};
+
+__device__ __forceinline__ void load_infer(const NetView &nv, int lane, InferRegs &r)
+{
+    const int c = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; t++) r.w1[t] = ld4(nv.w1b + (16 * t + c) * 16 + 4 * g);
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+        for (int tm = 0; tm < 4; tm++) {
+            const unsigned short *row = nv.w2b + (16 * tm + c) * kH + 32 * kk + 4 * g;
+            const v4s lo = ld4(row), hi = ld4(row + 16);
+            r.w2[kk][tm] = v8s{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            r.b1[t][i] = nv.params[oB1 + 16 * t + 4 * g + i];
+            r.b2[t][i] = nv.params[oB2 + 16 * t + 4 * g + i];
+            r.w3[t][i] = nv.params[oW3 + 16 * t + 4 * g + i];
+        }
+    r.b3 = nv.params[oB3];
+}
+
+// xb = X[feature 4g + j][sample c]; returns the output of sample c
+__device__ __forceinline__ float fwd_infer(const InferRegs &r, v4s xb)
+{
+    float h1[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const v4f z = mfma(r.w1[t], xb, v4f{r.b1[t][0], r.b1[t][1], r.b1[t][2], r.b1[t][3]});
+#pragma unroll
+        for (int i = 0; i < 4; i++) h1[t][i] = relu(z[i]);
+    }
+    v8s hp[2];
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+        const v4s lo = pack4(h1[2 * kk][0], h1[2 * kk][1], h1[2 * kk][2], h1[2 * kk][3]);
+        const v4s hi = pack4(h1[2 * kk + 1][0], h1[2 * kk + 1][1], h1[2 * kk + 1][2], h1[2 * kk + 1][3]);
+        hp[kk] = v8s{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int tm = 0; tm < 4; tm++) {
+        v4f z2 = v4f{r.b2[tm][0], r.b2[tm][1], r.b2[tm][2], r.b2[tm][3]};
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) z2 = mfma32(r.w2[kk][tm], hp[kk], z2);
+#pragma unroll
+        for (int i = 0; i < 4; i++) part = fmaf(r.w3[tm][i], relu(z2[i]), part);
+    }
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    return part + r.b3;
+}
+
+// asinh(x) = log(|x| + sqrt(x^2 + 1)) with the sign of x, on the hardware log2 and square root;
+// above 2^12, x^2 + 1 rounds to x^2 and the form log(2|x|) is used.  Its error (a few units of the
+// last place of log) is far below the BF16 rounding of the normalised feature it feeds.
+__device__ __forceinline__ float asinh_fast(float x)
+{
+    const float ax = fabsf(x);
+    const float y = ax > 4096.f ? 2.f * ax : ax + __builtin_sqrtf(fmaf(ax, ax, 1.f));
+    return copysignf(__builtin_amdgcn_logf(y) * 0.69314718055994531f, x);
 }
 
 __device__ __forceinline__ float pick4(const float o[4], int i)
@@ -411,21 +502,26 @@ apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef, floa
         wave_sync();
         // the network operands are (re)loaded after the features, so the two phases do not
         // hold their registers at the same time (cache hits after the first pass)
-        FwdRegs r;
-        load_fwd(nv, lane, r);
+        InferRegs r;
+        load_infer(nv, lane, r);
         const int tiles = 4 * (NT - tr0 < TRC ? NT - tr0 : TRC);
+        // tile st holds the samples 16 st + c, i.e. tracer st / 4 of lanes (c, st % 4); every lane
+        // keeps the outputs of its own samples
+        float om[TRC];
+#pragma unroll
+        for (int t = 0; t < TRC; t++) om[t] = 0.f;
 #pragma unroll 1
         for (int st = 0; st < tiles; st++) {
-            const v4s xb = ld4(xs + (16 * st + c) * 16 + 4 * g);
-            float h1[4][4], hh2[4][4], o[4];
-            fwd_tile(r, xb, h1, hh2, o);
-            if (c < 4) os[16 * st + 4 * g + c] = pick4(o, c);
+            const float o = fwd_infer(r, ld4(xs + (16 * st + c) * 16 + 4 * g));
+            const bool mine = g == (st & 3);
+#pragma unroll
+            for (int t = 0; t < TRC; t++)
+                if (mine && (st >> 2) == t) om[t] = o;
         }
-        wave_sync();
 #pragma unroll
         for (int t = 0; t < TRC; t++) {
             if (tr0 + t >= NT || !pok) continue;
-            float sv = softplus_f(os[t * 64 + lane]);
+            float sv = softplus_f(om[t]);
             if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
             if (sv > s_max) sv = s_max;
             if (a.sbound && sv > a.sbound[p]) sv = a.sbound[p];
@@ -435,6 +531,212 @@ apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef, floa
             else gamma[o] = hv_coef * (T)sv;
         }
         wave_sync();   // the LDS of the pass is rewritten by the next one
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation on 16-node tiles (FP32, 4 tracers of 64 levels), the form the FP32 time loop uses.
+// The five stencil operators of the features (Dx, Dy, Delta, Delta^2, Delta^k) are FP32 products
+// of dense 16 x U weight blocks with the tile's union columns on v_mfma_f32_16x16x4f32, laid out as
+// in the vector stage kernel: a wavefront per tracer, lane (c, g) loads levels 4c .. 4c + 3 of a
+// union column, so one 16-byte load feeds 4 level blocks x 5 operators and the lane ends with nodes
+// g + 4i at levels 4c .. 4c + 3.  The vertical derivative comes from the tile's own columns (the
+// first 16 union positions) by DPP row shifts over a 12-level window.  The column gathers of the
+// per-point form (55 neighbours x 5 operators of vector FMAs per point) made it vector-ALU bound.
+// Then, per chunk of 64 points (one per lane), the features go through the wavefront's LDS into
+// the network's operand layout and through the network as 4 tiles of 16.
+constexpr int kCOps = 5;
+constexpr int kNodeRec = 17;   // per tile node in LDS: velocity basis (16) and spacing h
+
+__host__ __device__ inline size_t cvec_lds_bytes(int up)
+{
+    const size_t wld = (size_t)up + 1;
+    const size_t wbytes = kCOps * 16 * wld * 4;              // dense weight blocks
+    const size_t nbytes = 4 * (size_t)64 * 16 * 2;           // after the products: per wavefront the BF16
+                                                              // features of 64 points
+    return (size_t)(16 * kNodeRec + kMaxLevels * kVecWin) * 4 + (size_t)up * 4 + (wbytes > nbytes ? wbytes : nbytes);
+}
+
+// 2 wavefronts per SIMD (256 registers: 80 accumulators, the prefetched columns, the network)
+template <bool S16>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+cvec_kernel(FeatArgs<float> a, NetView nv, MTilePlan tp, int up, float *gamma, _Float16 *s16, float hv_coef,
+            float s_max, int every)
+{
+    const StageArgs<float> &s = a.s;
+    // re-evaluate only on steps that are multiples of `every` (as the per-point form)
+    if ((*s.base + s.step_off + *a.origin) % every != 0) return;
+    extern __shared__ __align__(16) unsigned char cv_lds[];
+    constexpr int S = 64, NTR = 4, NTH = 256;
+    const int wld = up + 1;
+    float *nrec = (float *)cv_lds;                          // [16][kNodeRec]
+    float *Ct = nrec + 16 * kNodeRec;                        // [64][kVecWin]
+    int *uoff = (int *)(Ct + kMaxLevels * kVecWin);          // [up] byte offsets of the union columns
+    float *W = (float *)(uoff + up);                         // [kCOps][16][wld]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int tile = xcd_remap(blockIdx.x, s.nblk);
+    const int p0 = tile * 16;
+    const int U = tp.ucount[tile];
+    const int u16 = (U + 15) & ~15;
+    const int n = s.n;
+    const float *ts = s.tscal + ((size_t)(*s.base + s.step_off) * 4) * kTimeScalars;
+    const int *uni = tp.uni + (size_t)tile * tp.ustride;
+    const unsigned lo4 = (unsigned)(w * S + 4 * c) * 4u;
+
+    // own columns (union positions 0..15; in a last tile past N the union's other columns)
+    float4 own[4];
+    if (p0 + 16 <= s.N) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) own[j] = ld_at<float4>(s.qin, (unsigned)(p0 + 4 * j + g) * (NTR * S * 4u) + lo4);
+    } else {
+        const int nown = s.N - p0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = 4 * j + g;
+            const int id = k < nown ? p0 + k : uni[k < U ? k : 0];
+            own[j] = ld_at<float4>(s.qin, (unsigned)id * (NTR * S * 4u) + lo4);
+        }
+    }
+    // ---- set-up: node records, union offsets, vertical table, dense weight blocks ----
+    {
+        const int t = tid >> 4, k = tid & 15;   // 256 threads: one basis value each
+        const int p = p0 + t < s.N ? p0 + t : s.N - 1;
+        nrec[t * kNodeRec + k] = s.basis[(size_t)p * kBasis + k];
+        if (k == 0) nrec[t * kNodeRec + 16] = a.hloc[p];
+    }
+    for (int k = tid; k < u16; k += NTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S * 4);
+    build_vertical_table(Ct, s.Nv, s.nv, s.vstart, s.vW, tid, NTH);
+    for (int e = tid; e < kCOps * 16 * wld; e += NTH) W[e] = 0.f;
+    __syncthreads();
+    {
+        const int ne = (p0 + 16 <= s.N ? 16 : s.N - p0) * n;
+        const float rn = 1.f / (float)n;
+        const float *src[kCOps] = {s.Wx, s.Wy, a.WL1, a.WL2, s.WL};
+        for (int e = tid; e < ne; e += NTH) {
+            // (e + 1/2) / n is at least 1/(2n) away from an integer, far above the rounding
+            const int t = (int)(((float)e + 0.5f) * rn);
+            const size_t o = (size_t)p0 * n + e;
+            const int col = vec_row(t) * wld + tp.lidx[o];
+#pragma unroll
+            for (int op = 0; op < kCOps; op++) W[op * 16 * wld + col] = src[op][o];
+        }
+    }
+    __syncthreads();
+
+    // ---- D_op (16 nodes x 64 levels of tracer w) = W_op (16 x u16) x Q (u16 x 64) ----
+    v4f acc[kCOps][4];
+#pragma unroll
+    for (int op = 0; op < kCOps; op++)
+#pragma unroll
+        for (int lb = 0; lb < 4; lb++) acc[op][lb] = v4f{0.f, 0.f, 0.f, 0.f};
+    const float *wa = W + c * wld + g;
+    auto step = [&](int k0, const float4 (&b)[4]) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float av[kCOps];
+#pragma unroll
+            for (int op = 0; op < kCOps; op++) av[op] = wa[op * 16 * wld + k0 + 4 * j];
+            const float bv[4] = {b[j].x, b[j].y, b[j].z, b[j].w};
+#pragma unroll
+            for (int lb = 0; lb < 4; lb++)
+#pragma unroll
+                for (int op = 0; op < kCOps; op++) acc[op][lb] = mfma4(av[op], bv[lb], acc[op][lb]);
+        }
+    };
+    auto load_step = [&](int k0, float4 (&b)[4]) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = ld_at<float4>(s.qin, (unsigned)uoff[k0 + 4 * j + g] + lo4);
+    };
+    float4 bn[4];
+    if (u16 > 16) load_step(16, bn);
+    const int v0 = 4 * c;
+    float dz[4][4];   // [node i][level block]
+#pragma unroll
+    for (int i = 0; i < 4; i++) vertical4(Ct, v0, own[i], dz[i]);
+    __builtin_amdgcn_sched_barrier(0);
+    step(0, own);
+    for (int k0 = 16; k0 < u16; k0 += 32) {
+        float4 bm[4];
+        if (k0 + 16 < u16) load_step(k0 + 16, bm);
+        __builtin_amdgcn_sched_barrier(0);
+        step(k0, bn);
+        if (k0 + 16 >= u16) break;
+        if (k0 + 32 < u16) load_step(k0 + 32, bn);
+        __builtin_amdgcn_sched_barrier(0);
+        step(k0 + 16, bm);
+    }
+    __syncthreads();   // every wavefront is done with the weight blocks: the network staging reuses them
+
+    // ---- features and the network, 64 points (one per lane) at a time ----
+    unsigned short *xs = (unsigned short *)W + w * (64 * 16);             // [64][16] BF16
+    InferRegs r;
+    load_infer(nv, lane, r);
+    const float K = kKappa;
+#pragma unroll 1
+    for (int i = 0; i < 4; i++) {
+        const int t = g + 4 * i;
+        const int pr = p0 + t;
+        const int p = pr < s.N ? pr : s.N - 1;
+        const float *B = nrec + t * kNodeRec;
+        const float h = B[16], h2 = h * h, h4 = h2 * h2, h8 = h4 * h4;
+        const float ov[4] = {own[i].x, own[i].y, own[i].z, own[i].w};
+        float sv4[4];
+#pragma unroll
+        for (int lb = 0; lb < 4; lb++) {
+            const int v = v0 + lb;
+            const int vv = v < s.Nv ? v : s.Nv - 1;
+            const float gx = acc[0][lb][i], gy = acc[1][lb][i];
+            float ux, uy, uz;
+            level_velocity<float, S>(s, B, ts, p, vv, ux, uy, uz);
+            const float speed = sqrtf(ux * ux + uy * uy + uz * uz);
+            float f[kF];
+            f[0] = asinh_fast(K * h * sqrtf(gx * gx + gy * gy));
+            f[1] = asinh_fast(K * h2 * acc[2][lb][i]);
+            f[2] = asinh_fast(K * h4 * acc[3][lb][i]);
+            f[3] = asinh_fast(K * h8 * acc[4][lb][i]);
+            f[4] = h / a.hbar;
+            f[5] = speed * s.dt / (a.R * h);
+            f[6] = speed / a.umax;
+            f[7] = ((float)vv + 0.5f) / (float)s.Nv;
+            f[8] = ov[lb];
+            f[9] = asinh_fast(K * a.dh * fabsf(dz[i][lb]));
+            float xn[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) xn[k] = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
+            v4s *xrow = (v4s *)(xs + lane * 16);
+#pragma unroll
+            for (int k = 0; k < 4; k++) xrow[k] = pack4(xn[4 * k], xn[4 * k + 1], xn[4 * k + 2], xn[4 * k + 3]);
+            wave_sync();
+            // tile st holds the points of lanes (c, st); every lane keeps its own point's output
+            float om = 0.f;
+#pragma unroll 1
+            for (int st = 0; st < 4; st++) {
+                const float o = fwd_infer(r, ld4(xs + (16 * st + c) * 16 + 4 * g));
+                if (g == st) om = o;
+            }
+            float sv = softplus_f(om);
+            if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
+            if (sv > s_max) sv = s_max;
+            if (a.sbound && sv > a.sbound[p]) sv = a.sbound[p];
+            if (v >= s.Nv) sv = 0.f;
+            sv4[lb] = sv;
+            wave_sync();   // xs and os are rewritten by the next chunk
+        }
+        if (pr < s.N) {
+            const unsigned m = (unsigned)p * (NTR * S) + (unsigned)(w * S + v0);
+            if (S16) {
+                ushort4 hv;
+                hv.x = __builtin_bit_cast(unsigned short, (_Float16)sv4[0]);
+                hv.y = __builtin_bit_cast(unsigned short, (_Float16)sv4[1]);
+                hv.z = __builtin_bit_cast(unsigned short, (_Float16)sv4[2]);
+                hv.w = __builtin_bit_cast(unsigned short, (_Float16)sv4[3]);
+                st_at<ushort4>(s16, m * 2u, hv);
+            } else {
+                st_at<float4>(gamma, m * 4u, make_float4(hv_coef * sv4[0], hv_coef * sv4[1], hv_coef * sv4[2],
+                                                          hv_coef * sv4[3]));
+            }
+        }
     }
 }
 
@@ -799,10 +1101,53 @@ void launch_apply_s(serbf_engine *e, dim3 grid, size_t bytes, const FeatArgs<T> 
                        npad);
 }
 
+// the tile form of the evaluation: FP32, 4 tracers of 64 levels, config-mode flows, a tile plan
+// (the engine builds one for the tile stage kernels) whose weight blocks fit the LDS
+bool cvec_supported(const serbf_engine *e)
+{
+    return e->have_plan && mtile_supported(e) && e->ntr == 4 && vertical_window_fits(e) &&
+           (size_t)e->N * 4 * 64 * 4 < (1ull << 32) &&
+           cvec_lds_bytes((e->tile_umax + 15) & ~15) <= (size_t)160 * 1024;
+}
+
+template <bool S16>
+void launch_cvec_s(serbf_engine *e, const FeatArgs<float> &a, const NetView &nv, const MTilePlan &tp, int up,
+                   size_t bytes, float *gamma, _Float16 *s16, float hv, float smax, int every)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)cvec_kernel<S16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((cvec_kernel<S16>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, nv, tp, up, gamma, s16,
+                       hv, smax, every);
+}
+
+int launch_cvec(serbf_engine *e, const FeatArgs<float> &a0, float *gamma, int every, _Float16 *s16)
+{
+    FeatArgs<float> a = a0;
+    a.s.nblk = e->ntiles;
+    MTilePlan tp;
+    tp.uni = e->d_tuni;
+    tp.ucount = e->d_tcount;
+    tp.lidx = e->d_lidx;
+    tp.ustride = e->tile_ustride;
+    const int up = (e->tile_umax + 15) & ~15;
+    const size_t bytes = cvec_lds_bytes(up);
+    const NetView nv = view(e->closure);
+    if (s16) launch_cvec_s<true>(e, a, nv, tp, up, bytes, gamma, s16, (float)e->cfg.hv_coef, (float)e->closure_smax, every);
+    else launch_cvec_s<false>(e, a, nv, tp, up, bytes, gamma, s16, (float)e->cfg.hv_coef, (float)e->closure_smax, every);
+    SERBF_HIP(hipGetLastError());
+    return SERBF_OK;
+}
+
 template <typename T>
 // writes the coefficients to gamma, or (s16 != null) the multipliers in FP16 to s16
 int launch_apply(serbf_engine *e, const FeatArgs<T> &a, T *gamma, int every, _Float16 *s16 = nullptr)
 {
+    if constexpr (std::is_same<T, float>::value)
+        if (cvec_supported(e)) return launch_cvec(e, a, gamma, every, s16);
     const int npad = (e->n + 7) / 8 * 8;
     const int npw = 64 / e->S;
     const long passes = ((long)e->N + npw - 1) / npw * ((e->ntr + 3) / 4);
@@ -1131,6 +1476,10 @@ int serbf_closure_gamma(serbf_engine *e, int precision, double t, double dt, dou
     const size_t cols = (size_t)e->N * e->ntr, P = cols * S;
     int rc = SERBF_OK;
     if (precision == SERBF_F32) {
+        // the time loop's FP32 evaluation runs on the tile plan where one applies: build it here too
+        if (!e->have_plan && e->ntr == 4 && e->S == 64 && e->n <= kMaxStencil && e->cfg.mode == SERBF_MODE_CONFIG &&
+            (rc = tile_plan(e)))
+            return rc;
         float tsf[4 * kTimeScalars];
         for (int i = 0; i < 4 * kTimeScalars; i++) tsf[i] = (float)ts[i];
         DevTemp<float> d_ts;

@@ -20,6 +20,7 @@
 
 #include "engine.h"
 #include "stage_common.h"
+#include "tile_common.h"
 
 namespace serbf {
 
@@ -31,19 +32,6 @@ constexpr int kMTileNodes = 16;
 // 84 KB (FP64)
 constexpr int kMTileUcap = 320;
 
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ v4f mfma4(float a, float b, v4f c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-// v_mfma_f64_16x16x4f64: the same operand layout as the FP32 form; result value i of lane
-// (c, g) is D[g + 4i][c] (FP32: D[4g + i][c])
-__device__ __forceinline__ v4d mfma4(double a, double b, v4d c)
-{
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 template <typename T>
 struct Acc4;
 template <>
@@ -56,13 +44,6 @@ template <>
 struct Acc4<double> {
     typedef v4d type;
     __device__ static int row(int g, int i) { return g + 4 * i; }
-};
-
-struct MTilePlan {
-    const int *uni;              // [ntiles][ustride] global ids of the union
-    const int *ucount;           // [ntiles]
-    const unsigned short *lidx;  // [N][n] union position of each stencil entry
-    int ustride;
 };
 
 // ---------------------------------------------------------------------------
@@ -326,39 +307,11 @@ mtile2_kernel(StageArgs<T> a, MTilePlan tp, int up)
 // the zero ghosts below level 0 and above level 63), and a per-level table of 12 coefficients
 // over levels 4c - 4 .. 4c + 7 (built in LDS at set-up from the vertical stencil) covers the
 // one-sided rows at the ends.  The update's loads and stores are float4 per node.
-constexpr int kVecWin = 12;   // vertical window of a lane: levels 4c - 4 .. 4c + 7
-
 __host__ __device__ inline int mvec_lds_bytes(int up, bool hv)
 {
     const int wld = up + 1;
     return (32 + 16 * wld * (hv ? 2 : 1) + kMaxLevels * kVecWin) * 4 + up * 4;
 }
-
-// the value of lane c - 1 (c + 1) of the same 16-lane row, 0 for c = 0 (c = 15)
-__device__ __forceinline__ float row_prev(float x)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float row_next(float x)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x101, 0xF, 0xF, false));
-}
-
-// 16-byte accesses at a byte offset below 4 GiB from a kernel-argument base: the scalar base plus
-// a 32-bit offset register (global saddr form) instead of a 64-bit address per lane
-template <typename V, typename P>
-__device__ __forceinline__ V ld_at(const P *base, unsigned byte_off)
-{
-    return *(const V *)((const char *)base + byte_off);
-}
-template <typename V, typename P>
-__device__ __forceinline__ void st_at(P *base, unsigned byte_off, const V &v)
-{
-    *(V *)((char *)base + byte_off) = v;
-}
-
-// MFMA row of tile node t: node g + 4i lands in row 4g + i, i.e. in result value i of row group g
-__device__ __forceinline__ int vec_row(int t) { return 4 * (t & 3) + (t >> 2); }
 
 // 4 wavefronts per SIMD (128 registers), 4 work-groups of 4 wavefronts per CU.  HVM is the
 // hyperviscosity coefficient's source: 0 none, 1 the constant hv_coef, 2 hv_coef times the
@@ -432,15 +385,7 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
         sv[2 * tid + 1] = uy;
     }
     for (int k = tid; k < u16; k += NTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S * 4);   // bytes
-    for (int e = tid; e < kMaxLevels * kVecWin; e += NTH) {
-        const int l = e / kVecWin, r = e - l * kVecWin;
-        float cv = 0.f;
-        if (l < a.Nv) {
-            const int m = r - (a.vstart[l] - 4 * (l >> 2) + 4);
-            if (m >= 0 && m < a.nv) cv = a.vW[l * kMaxNv + m];
-        }
-        Ct[e] = cv;
-    }
+    build_vertical_table(Ct, a.Nv, a.nv, a.vstart, a.vW, tid, NTH);
     for (int e = tid; e < 16 * wld * (HV ? 2 : 1); e += NTH) W[e] = 0.f;
     __syncthreads();
     const float rn = 1.f / (float)n;
@@ -495,24 +440,10 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
                              wt * wprof.w / rinv.w};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const float ov[4] = {own[i].x, own[i].y, own[i].z, own[i].w};
-            float win[kVecWin];
+            float z[4];
+            vertical4(Ct, v0, own[i], z);
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                win[k] = row_prev(ov[k]);
-                win[4 + k] = ov[k];
-                win[8 + k] = row_next(ov[k]);
-            }
-#pragma unroll
-            for (int lb = 0; lb < 4; lb++) {
-                const float4 *cr = (const float4 *)(Ct + (v0 + lb) * kVecWin);
-                const float4 c0 = cr[0], c1 = cr[1], c2 = cr[2];
-                const float cf[kVecWin] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y, c2.z, c2.w};
-                float zv = 0.f;
-#pragma unroll
-                for (int r = 0; r < kVecWin; r++) zv = fmaf(cf[r], win[r], zv);
-                acc[lb][i] = sc[lb] * zv;
-            }
+            for (int lb = 0; lb < 4; lb++) acc[lb][i] = sc[lb] * z[lb];
         }
 #pragma unroll
         for (int lb = 0; lb < 4; lb++) accl[lb] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -637,13 +568,7 @@ int launch_mvec(serbf_engine *e, int stage, int hv, const StageArgs<float> &a, c
 bool mvec_supported(const serbf_engine *e)
 {
     // 4 tracers of 64 levels, state arrays below 4 GiB (32-bit byte offsets)
-    if (e->ntr != 4 || e->cfg.Nv > kMaxLevels || e->nv > kMaxNv || (size_t)e->N * 4 * 64 * 4 >= (1ull << 32))
-        return false;
-    for (int l = 0; l < e->cfg.Nv; l++) {
-        const int b = e->h_vstart[l] - 4 * (l >> 2) + 4;
-        if (b < 0 || b + e->nv > kVecWin) return false;
-    }
-    return true;
+    return e->ntr == 4 && (size_t)e->N * 4 * 64 * 4 < (1ull << 32) && vertical_window_fits(e);
 }
 
 // tracers per wavefront of the version-5 kernel: 2 (8 wavefronts per work-group).  Measured at

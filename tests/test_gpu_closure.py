@@ -167,7 +167,7 @@ def _oracle_gamma(small, params, norm, t, q, s_max=8.0, bound=None):
     feat, _ = O.closure_features(cfg, small["xyz"], small["idx"], *small["W"], small["vstart"], small["vW"], q, t,
                                  small["dt"])
     o = O.mlp_forward(params, norm, feat, emulate=True)
-    s = np.log1p(np.exp(np.minimum(o, 30.0)))
+    s = np.where(o > 20.0, o, np.log1p(np.exp(np.minimum(o, 20.0))))   # the kernels' softplus
     s = np.where(np.isfinite(s), np.minimum(s, s_max), 1.0).reshape(q.shape)
     if bound is not None:
         s = np.minimum(s, bound.astype(np.float64).reshape((-1,) + (1,) * (q.ndim - 1)))

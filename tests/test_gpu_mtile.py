@@ -215,3 +215,46 @@ def test_config_D_closure_fp16_multiplier_matches_oracle(deform60, kver):
         f4 = P.rhs(qo + dt * f3, t + dt, g64)
         qo = qo + dt / 6 * (((f1 + 2 * f2) + 2 * f3) + f4)
     assert rel_l2(q64, qo) < 1e-11
+
+
+def test_closure_tile_evaluation_matches_oracle(deform60):
+    """The FP32 closure evaluation on the tile plan (the five feature operators as matrix-core
+    products, the vertical derivative by row shifts, the network per 64 points) at the headline's
+    shape: its multipliers against the oracle's features and BF16 network on 20000 sampled points,
+    with the per-node stability bound, and against the per-point FP64 evaluation everywhere."""
+    d = deform60
+    eng, cfg, q0, dt = d["eng"], d["cfg"], d["q0"], d["dt"]
+    params, norm, rep = CL.load()
+    cl = E.Closure(0)
+    cl.set_params(params, norm)
+    t = 0.3 * cfg.T
+    try:
+        eng.set_state(q0)
+        eng.attach_closure(cl, every=16, s_max=rep["s_max"])
+        bound = eng.closure_bound(rep["kappa"])
+        g32 = eng.closure_gamma(F32, t, dt) / cfg.hv_coef
+        g64 = eng.closure_gamma(F64, t, dt) / cfg.hv_coef
+        idx, _ = eng.stencils()
+        Dx, Dy, L, _ = eng.get_weights()
+        L1, L2 = eng.lap_weights()
+        vstart, vW = eng.vertical()
+    finally:
+        eng.attach_closure(None)
+        cl.close()
+    feat, _ = O.closure_features(cfg, d["xyz"], idx, Dx, Dy, L, L1, L2, vstart, vW, q0, t, dt)
+    rng = np.random.default_rng(5)
+    pick = rng.choice(len(feat), 20000, replace=False)
+    o = O.mlp_forward(params, norm, feat[pick], emulate=True)
+    sref = np.minimum(np.where(o > 20.0, o, np.log1p(np.exp(np.minimum(o, 20.0)))), rep["s_max"])
+    node = pick // (4 * cfg.Nv)
+    sref = np.minimum(sref, bound[node].astype(np.float64))
+    got = g32.reshape(-1)[pick]
+    # the network's BF16 operands leave relative differences of a few 1e-3 in s (up to ~60 here)
+    rel = np.abs(got - sref) / np.maximum(1.0, sref)
+    rel64 = np.abs(g32 - g64) / np.maximum(1.0, g64)
+    print(f"tile evaluation vs oracle: share rel < 4e-3: {np.mean(rel < 4e-3):.4f}, max {rel.max():.2e}; "
+          f"vs FP64 per-point: share rel < 4e-3: {np.mean(rel64 < 4e-3):.4f}, max {rel64.max():.2e}")
+    assert np.mean(rel < 4e-3) > 0.99 and rel.max() < 2e-2
+    # FP32 and FP64 feature sums can flip a BF16 rounding of the network input at a few points
+    assert np.mean(rel64 < 4e-3) > 0.99 and rel64.max() < 0.1
+    assert np.all(g32 <= rep["s_max"]) and np.all(g32 <= bound[:, None, None] + 1e-3)
