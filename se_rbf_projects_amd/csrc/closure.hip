@@ -77,9 +77,10 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
 }
 __device__ __forceinline__ float softplus_f(float o) { return o > 20.f ? o : log1pf(expf(o)); }
 __device__ __forceinline__ float sigmoid_f(float o) { return 1.f / (1.f + expf(-o)); }
-// hidden-layer activation: ReLU, one full-rate instruction per value (the tanh it replaces took
-// two quarter-rate transcendentals, which made the hidden layers the evaluation's bottleneck)
-__device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
+// hidden-layer activation: ReLU as a signed-integer maximum of the bits, one full-rate
+// instruction (fmaxf(x, 0) takes two: IEEE mode makes it quiet a signalling NaN first).  Negative
+// values and -0 are negative integers and give +0; positive values and positive NaNs pass.
+__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -212,18 +213,19 @@ __device__ __forceinline__ void load_infer(const NetView &nv, int lane, InferReg
 // xb = X[feature 4g + j][sample c]; returns the output of sample c
 __device__ __forceinline__ float fwd_infer(const InferRegs &r, v4s xb)
 {
-    float h1[4][4];
+    // layer 1, ReLU applied to the BF16 pairs as a 16-bit signed maximum (rounding to BF16 and
+    // ReLU commute: a negative value rounds to a negative BF16 or -0)
+    v4s h1[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const v4f z = mfma(r.w1[t], xb, v4f{r.b1[t][0], r.b1[t][1], r.b1[t][2], r.b1[t][3]});
-#pragma unroll
-        for (int i = 0; i < 4; i++) h1[t][i] = relu(z[i]);
+        const v4s zb = pack4(z[0], z[1], z[2], z[3]);
+        h1[t] = __builtin_elementwise_max(zb, v4s{0, 0, 0, 0});
     }
     v8s hp[2];
 #pragma unroll
     for (int kk = 0; kk < 2; kk++) {
-        const v4s lo = pack4(h1[2 * kk][0], h1[2 * kk][1], h1[2 * kk][2], h1[2 * kk][3]);
-        const v4s hi = pack4(h1[2 * kk + 1][0], h1[2 * kk + 1][1], h1[2 * kk + 1][2], h1[2 * kk + 1][3]);
+        const v4s lo = h1[2 * kk], hi = h1[2 * kk + 1];
         hp[kk] = v8s{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
     }
     float part = 0.f;
@@ -235,8 +237,12 @@ __device__ __forceinline__ float fwd_infer(const InferRegs &r, v4s xb)
 #pragma unroll
         for (int i = 0; i < 4; i++) part = fmaf(r.w3[tm][i], relu(z2[i]), part);
     }
-    part += __shfl_xor(part, 16, 64);
-    part += __shfl_xor(part, 32, 64);
+    // sum over the four lane rows: row pairs by the 16-lane swap, then halves by the 32-lane swap
+    // (each lane adds the same two operands as a xor-16 / xor-32 shuffle would)
+    const auto p16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+    part = __uint_as_float(p16[0]) + __uint_as_float(p16[1]);
+    const auto p32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(part), __float_as_uint(part), false, false);
+    part = __uint_as_float(p32[0]) + __uint_as_float(p32[1]);
     return part + r.b3;
 }
 
@@ -548,43 +554,204 @@ apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef, floa
 constexpr int kCOps = 5;
 constexpr int kNodeRec = 17;   // per tile node in LDS: velocity basis (16) and spacing h
 
-__host__ __device__ inline size_t cvec_lds_bytes(int up)
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ v4f mfma32h(v8h a, v8h b, v4f c)
 {
-    const size_t wld = (size_t)up + 1;
-    const size_t wbytes = kCOps * 16 * wld * 4;              // dense weight blocks
-    const size_t nbytes = 4 * (size_t)64 * 16 * 2;           // after the products: per wavefront the BF16
-                                                              // features of 64 points
-    return (size_t)(16 * kNodeRec + kMaxLevels * kVecWin) * 4 + (size_t)up * 4 + (wbytes > nbytes ? wbytes : nbytes);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-// 2 wavefronts per SIMD (256 registers: 80 accumulators, the prefetched columns, the network)
+// x as hi + lo, two FP16 values (hi holds 11 significant bits, lo the next 11), saturated to the
+// FP16 range (NaN stays NaN)
+__device__ __forceinline__ void split16(float x, _Float16 &hi, _Float16 &lo)
+{
+    const float xs = x > 65504.f ? 65504.f : (x < -65504.f ? -65504.f : x);
+    hi = (_Float16)xs;
+    lo = (_Float16)(xs - (float)hi);
+}
+
+typedef __fp16 hp2 __attribute__((ext_vector_type(2)));
+
+// x0, x1 (|x| <= 1, or non-finite) as packed FP16 pairs hi and lo with x = hi + lo + O(2^-22 |x|):
+// hi rounded toward zero, the remainder x - hi exact in FP32, lo its FP16 rounding toward zero
+__device__ __forceinline__ void split2(float x0, float x1, unsigned &hi, unsigned &lo)
+{
+    const hp2 h = __builtin_amdgcn_cvt_pkrtz(x0, x1);
+    const hp2 l = __builtin_amdgcn_cvt_pkrtz(x0 - (float)h[0], x1 - (float)h[1]);
+    hi = __builtin_bit_cast(unsigned, h);
+    lo = __builtin_bit_cast(unsigned, l);
+}
+
+// softplus on the hardware exp2 / log2: max(o, 0) + log(1 + e^-|o|) (for e^-|o| below 2^-24 the
+// second term rounds to 0, an absolute error below 6e-8)
+__device__ __forceinline__ float softplus_fast(float o)
+{
+    const float e = __builtin_amdgcn_exp2f(-fabsf(o) * 1.4426950408889634f);
+    return fmaxf(o, 0.f) + __builtin_amdgcn_logf(1.f + e) * 0.69314718055994531f;
+}
+
+// union positions padded to the 32 columns of a product step
+__host__ __device__ inline int cvec_u32(int up) { return (up + 31) & ~31; }
+
+__host__ __device__ inline size_t cvec_lds_bytes(int up)
+{
+    const size_t wld = (size_t)cvec_u32(up) + 8;               // FP16 row stride (16 bytes of padding)
+    const size_t wbytes = 2 * kCOps * 16 * wld * 2;             // hi and lo weight blocks
+    const size_t nbytes = 4 * (size_t)64 * 16 * 2;              // after the products: per wavefront the BF16
+                                                                 // features of 64 points
+    return (size_t)(16 * kNodeRec + kMaxLevels * kVecWin) * 4 + (size_t)cvec_u32(up) * 4 +
+           (wbytes > nbytes ? wbytes : nbytes);
+}
+
+// Per node and tracer the maximum of |q| over the levels of the FP32 state [N][4][64] (the
+// padding levels hold zeros), for the power-of-two scale of a tile's FP16 products: 16 lanes per
+// (node, tracer), one 16-byte load each.  It runs on the evaluation steps of the cadence only, as
+// the closure kernel.
+__global__ void __launch_bounds__(256) nodemax_kernel(FeatArgs<float> a, float *nmax, int every)
+{
+    const StageArgs<float> &s = a.s;
+    if ((*s.base + s.step_off + *a.origin) % every != 0) return;
+    // grid stride over whole wavefronts (N * 64 is a multiple of 64): every lane of a wavefront
+    // takes part in each shuffle
+    const size_t n4 = (size_t)s.N * 64;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) {
+        const float4 v = ((const float4 *)s.qin)[i];
+        float m = fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w)));
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 15) == 0) nmax[i >> 4] = m;
+    }
+}
+
+// 2 wavefronts per SIMD (256 registers: 80 accumulators, the column loads of the next step and
+// the split operands of this one, then the network)
 template <bool S16>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-cvec_kernel(FeatArgs<float> a, NetView nv, MTilePlan tp, int up, float *gamma, _Float16 *s16, float hv_coef,
-            float s_max, int every)
+cvec_kernel(FeatArgs<float> a, NetView nv, MTilePlan tp, int up, const float *nmax, float *gamma, _Float16 *s16,
+            float hv_coef, float s_max, int every)
 {
     const StageArgs<float> &s = a.s;
     // re-evaluate only on steps that are multiples of `every` (as the per-point form)
     if ((*s.base + s.step_off + *a.origin) % every != 0) return;
     extern __shared__ __align__(16) unsigned char cv_lds[];
     constexpr int S = 64, NTR = 4, NTH = 256;
-    const int wld = up + 1;
+    const int wld = cvec_u32(up) + 8;
     float *nrec = (float *)cv_lds;                          // [16][kNodeRec]
     float *Ct = nrec + 16 * kNodeRec;                        // [64][kVecWin]
-    int *uoff = (int *)(Ct + kMaxLevels * kVecWin);          // [up] byte offsets of the union columns
-    float *W = (float *)(uoff + up);                         // [kCOps][16][wld]
+    int *uoff = (int *)(Ct + kMaxLevels * kVecWin);          // [u32] byte offsets of the union columns
+    _Float16 *Wh = (_Float16 *)(uoff + cvec_u32(up));        // [kCOps][16][wld] weights, hi part
+    _Float16 *Wl = Wh + kCOps * 16 * wld;                    // [kCOps][16][wld] lo part
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int c = lane & 15, g = lane >> 4;
     const int tile = xcd_remap(blockIdx.x, s.nblk);
     const int p0 = tile * 16;
     const int U = tp.ucount[tile];
-    const int u16 = (U + 15) & ~15;
+    const int u32 = (U + 31) & ~31;
     const int n = s.n;
     const float *ts = s.tscal + ((size_t)(*s.base + s.step_off) * 4) * kTimeScalars;
     const int *uni = tp.uni + (size_t)tile * tp.ustride;
     const unsigned lo4 = (unsigned)(w * S + 4 * c) * 4u;
 
-    // own columns (union positions 0..15; in a last tile past N the union's other columns)
+    // ---- set-up: node records, union offsets, vertical table, dense weight blocks ----
+    {
+        const int t = tid >> 4, k = tid & 15;   // 256 threads: one basis value each
+        const int p = p0 + t < s.N ? p0 + t : s.N - 1;
+        nrec[t * kNodeRec + k] = s.basis[(size_t)p * kBasis + k];
+        if (k == 0) nrec[t * kNodeRec + 16] = a.hloc[p];
+    }
+    for (int k = tid; k < u32; k += NTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S * 4);
+    build_vertical_table(Ct, s.Nv, s.nv, s.vstart, s.vW, tid, NTH);
+    for (int e = tid; e < kCOps * 16 * wld; e += NTH) ((unsigned *)Wh)[e] = 0u;   // hi and lo
+    __syncthreads();
+    {
+        // the operators scaled to h, h, h^2, h^4, h^8 of their row's node (dimensionless, so
+        // FP16 holds them: row sums of a few thousand at most on quasi-uniform node sets)
+        const int ne = (p0 + 16 <= s.N ? 16 : s.N - p0) * n;
+        const float rn = 1.f / (float)n;
+        const float *src[kCOps] = {s.Wx, s.Wy, a.WL1, a.WL2, s.WL};
+        for (int e = tid; e < ne; e += NTH) {
+            // (e + 1/2) / n is at least 1/(2n) away from an integer, far above the rounding
+            const int t = (int)(((float)e + 0.5f) * rn);
+            const size_t o = (size_t)p0 * n + e;
+            const int col = vec_row(t) * wld + tp.lidx[o];
+            const float h = nrec[t * kNodeRec + 16], h2 = h * h, h4 = h2 * h2;
+            const float hk[kCOps] = {h, h, h2, h4, h4 * h4};
+#pragma unroll
+            for (int op = 0; op < kCOps; op++) split16(src[op][o] * hk[op], Wh[op * 16 * wld + col], Wl[op * 16 * wld + col]);
+        }
+    }
+    // The tracer's power-of-two scale over the tile's union, 2^-E with |q| <= 2^E (qm = f 2^E,
+    // 1/2 <= f < 1): the products then keep about 22 bits relative to the largest value of each
+    // sum, as FP32 sums do, and values below 2^-14 of it lose bits into FP16 subnormals.
+    float qm = 0.f;
+    for (int k = lane; k < U; k += 64) qm = fmaxf(qm, nmax[(size_t)uni[k] * 4 + w]);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) qm = fmaxf(qm, __shfl_xor(qm, o, 64));
+    float sig = 1.f;
+    if (qm > 0.f && qm <= 3.0e38f) {
+        int E;
+        (void)frexpf(qm, &E);
+        sig = ldexpf(1.f, -E);
+    }
+    __syncthreads();
+
+    // ---- D_op (16 nodes x 64 levels of tracer w) = W_op (16 x u32) x Q (u32 x 64) ----
+    // Each product is hi(W) hi(Q) + hi(W) lo(Q) + lo(W) hi(Q) on v_mfma_f32_16x16x32_f16 with FP32
+    // accumulation (about 22 significant bits per term).  Lane (c, g) holds the union columns
+    // k0 + 8g .. k0 + 8g + 7 of a 32-column step: the A operand row c from LDS, the B operand
+    // levels 4c + lb of the 8 columns from 8 16-byte loads.
+    v4f acc[kCOps][4];
+#pragma unroll
+    for (int op = 0; op < kCOps; op++)
+#pragma unroll
+        for (int lb = 0; lb < 4; lb++) acc[op][lb] = v4f{0.f, 0.f, 0.f, 0.f};
+    const _Float16 *wah = Wh + c * wld + 8 * g;
+    const _Float16 *wal = Wl + c * wld + 8 * g;
+    float4 b[8];
+    auto load_step = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) b[j] = ld_at<float4>(s.qin, (unsigned)uoff[k0 + 8 * g + j] + lo4);
+    };
+    // the scaled values are at most 1 in magnitude: no saturation needed
+    auto split_step = [&](v8h (&bh)[4], v8h (&bl)[4]) {
+        unsigned uh[4][4], ul[4][4];   // [level block][column pair]
+#pragma unroll
+        for (int jp = 0; jp < 4; jp++) {
+            const float4 x0 = b[2 * jp], x1 = b[2 * jp + 1];
+            split2(x0.x * sig, x1.x * sig, uh[0][jp], ul[0][jp]);
+            split2(x0.y * sig, x1.y * sig, uh[1][jp], ul[1][jp]);
+            split2(x0.z * sig, x1.z * sig, uh[2][jp], ul[2][jp]);
+            split2(x0.w * sig, x1.w * sig, uh[3][jp], ul[3][jp]);
+        }
+#pragma unroll
+        for (int lb = 0; lb < 4; lb++) {
+            bh[lb] = __builtin_bit_cast(v8h, uh[lb]);
+            bl[lb] = __builtin_bit_cast(v8h, ul[lb]);
+        }
+    };
+    auto step = [&](int k0, const v8h (&bh)[4], const v8h (&bl)[4]) {
+#pragma unroll
+        for (int op = 0; op < kCOps; op++) {
+            const v8h ah = *(const v8h *)(wah + op * 16 * wld + k0);
+            const v8h al = *(const v8h *)(wal + op * 16 * wld + k0);
+#pragma unroll
+            for (int lb = 0; lb < 4; lb++) {
+                acc[op][lb] = mfma32h(al, bh[lb], acc[op][lb]);
+                acc[op][lb] = mfma32h(ah, bl[lb], acc[op][lb]);
+                acc[op][lb] = mfma32h(ah, bh[lb], acc[op][lb]);
+            }
+        }
+    };
+    load_step(0);
+    for (int k0 = 0; k0 < u32; k0 += 32) {
+        v8h bh[4], bl[4];
+        split_step(bh, bl);
+        if (k0 + 32 < u32) load_step(k0 + 32);
+        __builtin_amdgcn_sched_barrier(0);
+        step(k0, bh, bl);
+    }
+    // own columns after the products (they come from L2 by then): the vertical derivative and the
+    // state feature (in a last tile past N the union's other columns, whose results are not stored)
     float4 own[4];
     if (p0 + 16 <= s.N) {
 #pragma unroll
@@ -598,88 +765,26 @@ cvec_kernel(FeatArgs<float> a, NetView nv, MTilePlan tp, int up, float *gamma, _
             own[j] = ld_at<float4>(s.qin, (unsigned)id * (NTR * S * 4u) + lo4);
         }
     }
-    // ---- set-up: node records, union offsets, vertical table, dense weight blocks ----
-    {
-        const int t = tid >> 4, k = tid & 15;   // 256 threads: one basis value each
-        const int p = p0 + t < s.N ? p0 + t : s.N - 1;
-        nrec[t * kNodeRec + k] = s.basis[(size_t)p * kBasis + k];
-        if (k == 0) nrec[t * kNodeRec + 16] = a.hloc[p];
-    }
-    for (int k = tid; k < u16; k += NTH) uoff[k] = (k < U ? uni[k] : uni[0]) * (NTR * S * 4);
-    build_vertical_table(Ct, s.Nv, s.nv, s.vstart, s.vW, tid, NTH);
-    for (int e = tid; e < kCOps * 16 * wld; e += NTH) W[e] = 0.f;
-    __syncthreads();
-    {
-        const int ne = (p0 + 16 <= s.N ? 16 : s.N - p0) * n;
-        const float rn = 1.f / (float)n;
-        const float *src[kCOps] = {s.Wx, s.Wy, a.WL1, a.WL2, s.WL};
-        for (int e = tid; e < ne; e += NTH) {
-            // (e + 1/2) / n is at least 1/(2n) away from an integer, far above the rounding
-            const int t = (int)(((float)e + 0.5f) * rn);
-            const size_t o = (size_t)p0 * n + e;
-            const int col = vec_row(t) * wld + tp.lidx[o];
-#pragma unroll
-            for (int op = 0; op < kCOps; op++) W[op * 16 * wld + col] = src[op][o];
-        }
-    }
-    __syncthreads();
-
-    // ---- D_op (16 nodes x 64 levels of tracer w) = W_op (16 x u16) x Q (u16 x 64) ----
-    v4f acc[kCOps][4];
-#pragma unroll
-    for (int op = 0; op < kCOps; op++)
-#pragma unroll
-        for (int lb = 0; lb < 4; lb++) acc[op][lb] = v4f{0.f, 0.f, 0.f, 0.f};
-    const float *wa = W + c * wld + g;
-    auto step = [&](int k0, const float4 (&b)[4]) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            float av[kCOps];
-#pragma unroll
-            for (int op = 0; op < kCOps; op++) av[op] = wa[op * 16 * wld + k0 + 4 * j];
-            const float bv[4] = {b[j].x, b[j].y, b[j].z, b[j].w};
-#pragma unroll
-            for (int lb = 0; lb < 4; lb++)
-#pragma unroll
-                for (int op = 0; op < kCOps; op++) acc[op][lb] = mfma4(av[op], bv[lb], acc[op][lb]);
-        }
-    };
-    auto load_step = [&](int k0, float4 (&b)[4]) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) b[j] = ld_at<float4>(s.qin, (unsigned)uoff[k0 + 4 * j + g] + lo4);
-    };
-    float4 bn[4];
-    if (u16 > 16) load_step(16, bn);
     const int v0 = 4 * c;
     float dz[4][4];   // [node i][level block]
 #pragma unroll
     for (int i = 0; i < 4; i++) vertical4(Ct, v0, own[i], dz[i]);
-    __builtin_amdgcn_sched_barrier(0);
-    step(0, own);
-    for (int k0 = 16; k0 < u16; k0 += 32) {
-        float4 bm[4];
-        if (k0 + 16 < u16) load_step(k0 + 16, bm);
-        __builtin_amdgcn_sched_barrier(0);
-        step(k0, bn);
-        if (k0 + 16 >= u16) break;
-        if (k0 + 32 < u16) load_step(k0 + 32, bn);
-        __builtin_amdgcn_sched_barrier(0);
-        step(k0 + 16, bm);
-    }
     __syncthreads();   // every wavefront is done with the weight blocks: the network staging reuses them
 
     // ---- features and the network, 64 points (one per lane) at a time ----
-    unsigned short *xs = (unsigned short *)W + w * (64 * 16);             // [64][16] BF16
+    unsigned short *xs = (unsigned short *)Wh + w * (64 * 16);            // [64][16] BF16
     InferRegs r;
     load_infer(nv, lane, r);
-    const float K = kKappa;
+    const float K = kKappa, Kq = kKappa / sig;   // the products carry h^k and the scale sig
+    const float ihbar = 1.f / a.hbar, iumax = 1.f / a.umax, inv = 1.f / (float)s.Nv;
 #pragma unroll 1
     for (int i = 0; i < 4; i++) {
         const int t = g + 4 * i;
         const int pr = p0 + t;
         const int p = pr < s.N ? pr : s.N - 1;
         const float *B = nrec + t * kNodeRec;
-        const float h = B[16], h2 = h * h, h4 = h2 * h2, h8 = h4 * h4;
+        const float h = B[16];
+        const float cfl = s.dt / (a.R * h);
         const float ov[4] = {own[i].x, own[i].y, own[i].z, own[i].w};
         float sv4[4];
 #pragma unroll
@@ -691,14 +796,14 @@ cvec_kernel(FeatArgs<float> a, NetView nv, MTilePlan tp, int up, float *gamma, _
             level_velocity<float, S>(s, B, ts, p, vv, ux, uy, uz);
             const float speed = sqrtf(ux * ux + uy * uy + uz * uz);
             float f[kF];
-            f[0] = asinh_fast(K * h * sqrtf(gx * gx + gy * gy));
-            f[1] = asinh_fast(K * h2 * acc[2][lb][i]);
-            f[2] = asinh_fast(K * h4 * acc[3][lb][i]);
-            f[3] = asinh_fast(K * h8 * acc[4][lb][i]);
-            f[4] = h / a.hbar;
-            f[5] = speed * s.dt / (a.R * h);
-            f[6] = speed / a.umax;
-            f[7] = ((float)vv + 0.5f) / (float)s.Nv;
+            f[0] = asinh_fast(Kq * sqrtf(gx * gx + gy * gy));
+            f[1] = asinh_fast(Kq * acc[2][lb][i]);
+            f[2] = asinh_fast(Kq * acc[3][lb][i]);
+            f[3] = asinh_fast(Kq * acc[4][lb][i]);
+            f[4] = h * ihbar;
+            f[5] = speed * cfl;
+            f[6] = speed * iumax;
+            f[7] = ((float)vv + 0.5f) * inv;
             f[8] = ov[lb];
             f[9] = asinh_fast(K * a.dh * fabsf(dz[i][lb]));
             float xn[16];
@@ -715,7 +820,7 @@ cvec_kernel(FeatArgs<float> a, NetView nv, MTilePlan tp, int up, float *gamma, _
                 const float o = fwd_infer(r, ld4(xs + (16 * st + c) * 16 + 4 * g));
                 if (g == st) om = o;
             }
-            float sv = softplus_f(om);
+            float sv = softplus_fast(om);
             if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
             if (sv > s_max) sv = s_max;
             if (a.sbound && sv > a.sbound[p]) sv = a.sbound[p];
@@ -1039,6 +1144,7 @@ FeatArgs<T> feat_args(serbf_engine *e, StateBuffers<T> &sb, const T *q, double d
 int ensure_hloc(serbf_engine *e)
 {
     if (!e->d_hloc) SERBF_HIP(hipMalloc((void **)&e->d_hloc, (size_t)e->N * sizeof(float)));
+    if (!e->d_nmax) SERBF_HIP(hipMalloc((void **)&e->d_nmax, (size_t)e->N * 4 * sizeof(float)));
     hipLaunchKernelGGL(hloc_kernel, dim3((e->N + 255) / 256), dim3(256), 0, e->stream, e->N, e->n, e->d_x, e->d_y,
                        e->d_z, e->d_idx, e->d_hloc);
     SERBF_HIP(hipGetLastError());
@@ -1107,7 +1213,7 @@ bool cvec_supported(const serbf_engine *e)
 {
     return e->have_plan && mtile_supported(e) && e->ntr == 4 && vertical_window_fits(e) &&
            (size_t)e->N * 4 * 64 * 4 < (1ull << 32) &&
-           cvec_lds_bytes((e->tile_umax + 15) & ~15) <= (size_t)160 * 1024;
+           cvec_lds_bytes((e->tile_umax + 15) & ~15) <= (size_t)160 * 1024 && e->d_nmax;
 }
 
 template <bool S16>
@@ -1120,8 +1226,11 @@ void launch_cvec_s(serbf_engine *e, const FeatArgs<float> &a, const NetView &nv,
                                   160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((cvec_kernel<S16>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, nv, tp, up, gamma, s16,
-                       hv, smax, every);
+    const size_t nb = ((size_t)e->N * 64 + 255) / 256;
+    hipLaunchKernelGGL(nodemax_kernel, dim3((unsigned)std::min<size_t>(nb, 4096)), dim3(256), 0, e->stream, a, e->d_nmax,
+                       every);
+    hipLaunchKernelGGL((cvec_kernel<S16>), dim3(e->ntiles), dim3(256), bytes, e->stream, a, nv, tp, up,
+                       (const float *)e->d_nmax, gamma, s16, hv, smax, every);
 }
 
 int launch_cvec(serbf_engine *e, const FeatArgs<float> &a0, float *gamma, int every, _Float16 *s16)

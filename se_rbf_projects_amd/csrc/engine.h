@@ -158,6 +158,7 @@ struct serbf_engine {
     double closure_kappa = 0.0;  // per-node stability bound of the multiplier (0: none)
     float *d_sbound = nullptr;   // [N] kappa G_ref / G_p
     float *d_hloc = nullptr;   // nearest-neighbour chord per node (closure features)
+    float *d_nmax = nullptr;   // [N][4] per node and tracer max |q| (closure tile evaluation)
     int cus = 0;               // compute units of the device (0 until queried)
 };
 

@@ -171,6 +171,8 @@ static void free_nodes(serbf_engine *e)
     clear_sizes(e);
     if (e->d_hloc) (void)hipFree(e->d_hloc);
     e->d_hloc = nullptr;
+    if (e->d_nmax) (void)hipFree(e->d_nmax);
+    e->d_nmax = nullptr;
     if (e->d_sbound) (void)hipFree(e->d_sbound);
     e->d_sbound = nullptr;
     e->closure_kappa = 0.0;

@@ -217,13 +217,22 @@ def test_config_D_closure_fp16_multiplier_matches_oracle(deform60, kver):
     assert rel_l2(q64, qo) < 1e-11
 
 
-def test_closure_tile_evaluation_matches_oracle(deform60):
-    """The FP32 closure evaluation on the tile plan (the five feature operators as matrix-core
-    products, the vertical derivative by row shifts, the network per 64 points) at the headline's
-    shape: its multipliers against the oracle's features and BF16 network on 20000 sampled points,
-    with the per-node stability bound, and against the per-point FP64 evaluation everywhere."""
+@pytest.mark.parametrize("scale", [1.0, 1.0e-6, 3.0e5])
+def test_closure_tile_evaluation_matches_oracle(deform60, scale):
+    """The FP32 closure evaluation on the tile plan (the five feature operators as split-FP16
+    matrix-core products, the vertical derivative by row shifts, the network per 64 points) at the
+    headline's shape: its multipliers against the oracle's features and BF16 network on 20000
+    sampled points, with the per-node stability bound, and against the per-point FP64 evaluation
+    everywhere.  The state scaled by 1e-6 (FP16 subnormals without the tile's power-of-two scale)
+    and by 3e5 (past the FP16 range) checks that scale.  At 3e5 the features themselves are
+    ill-conditioned in any FP32 arithmetic: they are asinh of K h^k D q, so on the constant plateaus
+    of the slotted cylinders, where D q is exactly 0, FP32 rounding of the sums (2^-24 of 3e5 times
+    the row sums of h^8 Delta^4, up to ~2e3) becomes arguments of order 100, an error of up to ~14
+    in the normalised input where plain FP32 products are emulated in numpy.  There the test
+    asks for finite multipliers within the clamps and agreement at 95% of the points."""
     d = deform60
-    eng, cfg, q0, dt = d["eng"], d["cfg"], d["q0"], d["dt"]
+    eng, cfg, dt = d["eng"], d["cfg"], d["dt"]
+    q0 = d["q0"] * scale
     params, norm, rep = CL.load()
     cl = E.Closure(0)
     cl.set_params(params, norm)
@@ -252,9 +261,13 @@ def test_closure_tile_evaluation_matches_oracle(deform60):
     # the network's BF16 operands leave relative differences of a few 1e-3 in s (up to ~60 here)
     rel = np.abs(got - sref) / np.maximum(1.0, sref)
     rel64 = np.abs(g32 - g64) / np.maximum(1.0, g64)
-    print(f"tile evaluation vs oracle: share rel < 4e-3: {np.mean(rel < 4e-3):.4f}, max {rel.max():.2e}; "
+    print(f"scale {scale:g}: tile evaluation vs oracle: share rel < 4e-3: {np.mean(rel < 4e-3):.4f}, max {rel.max():.2e}; "
           f"vs FP64 per-point: share rel < 4e-3: {np.mean(rel64 < 4e-3):.4f}, max {rel64.max():.2e}")
+    assert np.all(np.isfinite(g32)) and np.all(g32 >= 0.0)
+    assert np.all(g32 <= rep["s_max"]) and np.all(g32 <= bound[:, None, None] + 1e-3)
+    if scale > 1.0:
+        assert np.mean(rel < 4e-3) > 0.95 and np.mean(rel64 < 4e-3) > 0.95
+        return
     assert np.mean(rel < 4e-3) > 0.99 and rel.max() < 2e-2
     # FP32 and FP64 feature sums can flip a BF16 rounding of the network input at a few points
     assert np.mean(rel64 < 4e-3) > 0.99 and rel64.max() < 0.1
-    assert np.all(g32 <= rep["s_max"]) and np.all(g32 <= bound[:, None, None] + 1e-3)
