@@ -461,10 +461,10 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const unsigned m = myi(i);
-            if constexpr (HVM == 2) Hv[i] = ld_at<ushort4>(a.s16, m * 2u);
+            if constexpr (HVM == 2) Hv[i] = ld_nt_h4(a.s16, m * 2u);
             if constexpr (HVM == 3) Gv[i] = ld_at<float4>(a.gamma, m * 4u);
-            Fv[i] = STAGE >= 2 ? ld_at<float4>(a.F, m * 4u) : make_float4(0.f, 0.f, 0.f, 0.f);
-            Qv[i] = STAGE >= 1 ? ld_at<float4>(a.qT, m * 4u) : make_float4(0.f, 0.f, 0.f, 0.f);
+            Fv[i] = STAGE >= 2 ? ld_nt4(a.F, m * 4u) : make_float4(0.f, 0.f, 0.f, 0.f);
+            Qv[i] = STAGE >= 1 ? ld_nt4(a.qT, m * 4u) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
     };
     __builtin_amdgcn_sched_barrier(0);
@@ -518,8 +518,8 @@ mvec_kernel(StageArgs<float> a, MTilePlan tp, int up)
             }
         }
         const unsigned m = myi(i) * 4u;
-        if (STAGE <= 3) st_at<float4>(a.F, m, make_float4(fo[0], fo[1], fo[2], fo[3]));
-        if (STAGE >= 1) st_at<float4>(a.qout, m, make_float4(qo[0], qo[1], qo[2], qo[3]));
+        if (STAGE <= 3) st_nt4(a.F, m, make_float4(fo[0], fo[1], fo[2], fo[3]));
+        if (STAGE >= 1) st_nt4(a.qout, m, make_float4(qo[0], qo[1], qo[2], qo[3]));
     }
 }
 

@@ -54,6 +54,24 @@ __device__ __forceinline__ void st_at(P *base, unsigned byte_off, const V &v)
     *(V *)((char *)base + byte_off) = v;
 }
 
+// Streaming forms for data a stage touches once (the RK4 update's inputs and outputs): the
+// non-temporal hint keeps them from displacing the union columns, which the neighbouring tiles
+// read again, from L2.
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 ld_nt4(const float *base, unsigned byte_off)
+{
+    const v4f v = __builtin_nontemporal_load((const v4f *)((const char *)base + byte_off));
+    return make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ ushort4 ld_nt_h4(const _Float16 *base, unsigned byte_off)
+{
+    return __builtin_bit_cast(ushort4, __builtin_nontemporal_load((const v2u *)((const char *)base + byte_off)));
+}
+__device__ __forceinline__ void st_nt4(float *base, unsigned byte_off, float4 x)
+{
+    __builtin_nontemporal_store(v4f{x.x, x.y, x.z, x.w}, (v4f *)((char *)base + byte_off));
+}
+
 // MFMA row of tile node t: node g + 4i lands in row 4g + i, i.e. in result value i of row group g
 __device__ __forceinline__ int vec_row(int t) { return 4 * (t & 3) + (t >> 2); }
 

@@ -6,10 +6,14 @@
 
 `trace` prints per-kernel call counts and average durations from a
 `rocprofv3 --kernel-trace --stats` run; `pmc` prints per-kernel averages of the
-counters of one or more `rocprofv3 --pmc ...` runs; `traffic` turns a FETCH_SIZE run and
-a WRITE_SIZE run into HBM bytes per launch, each counter scaled by what it reads for the
-plain 16-byte stream-copy kernel of the same run (which moves copy-MiB in each direction).  Both read the rocpd SQLite
-database rocprofv3 writes (the `kernels` / `counters_collection` views).
+counters of one or more `rocprofv3 --pmc ...` runs; `traffic` turns a read run and a
+WRITE_SIZE run into HBM bytes per launch.  The read run is either TCC_EA0_RDREQ_128B_sum,
+TCC_EA0_RDREQ_64B_sum and TCC_EA0_RDREQ_32B_sum (L2 read requests to memory by size: bytes
+without a calibration; the plain 16-byte stream-copy kernel of the same run, which moves copy-MiB
+in each direction, is then a check), or FETCH_SIZE, scaled by what it reads for that copy kernel
+(on gfx950 FETCH_SIZE counts a 128-byte request as 64 bytes, since its TCC_BUBBLE term reads 0).
+WRITE_SIZE is scaled by the copy's figure (1.0 on gfx950).  All read the rocpd SQLite database
+rocprofv3 writes (the `kernels` / `counters_collection` views).
 """
 import collections
 import glob
@@ -82,22 +86,47 @@ def counter_stats(paths):
     return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
 
 
+_RDREQ = (("TCC_EA0_RDREQ_128B", 128), ("TCC_EA0_RDREQ_64B", 64), ("TCC_EA0_RDREQ_32B", 32))
+
+
+def _counter(d, name):
+    for k in (name + "_sum", name):
+        if k in d:
+            return d[k]
+    return None
+
+
+def read_bytes(d):
+    """Bytes one dispatch read from memory, from its L2 read requests counted by size, or None
+    when the run did not count them."""
+    v = [_counter(d, n) for n, _ in _RDREQ]
+    if any(x is None for x in v):
+        return None
+    return float(sum(x * size for x, (_, size) in zip(v, _RDREQ)))
+
+
 def traffic(fetch_dir, write_dir, copy_mib):
-    """{kernel: {fetch_KiB, write_KiB, hbm_bytes_per_launch}} with the counters calibrated on
-    stream_copy_kernel, and the calibration factors."""
+    """{kernel: {fetch_KiB, write_KiB, hbm_bytes_per_launch}} and how the read side was counted:
+    read requests by size, checked on stream_copy_kernel, or FETCH_SIZE calibrated on it."""
     f = counter_stats([fetch_dir])
     w = counter_stats([write_dir])
     # the plain 16-byte copy (one vector per thread, ordinary loads and stores)
     copy = next(k for k in ("stream_copy_kernel<1, false>", "stream_copy_kernel") if k in f and k in w)
     nbytes = copy_mib * 1024 * 1024
-    kf = nbytes / (f[copy]["FETCH_SIZE"] * 1024.0)
+    by_request = read_bytes(f[copy]) is not None
+    if by_request:
+        kf = 1.0
+        fetch = {k: read_bytes(d) for k, d in f.items()}
+    else:
+        kf = nbytes / (f[copy]["FETCH_SIZE"] * 1024.0)
+        fetch = {k: kf * d["FETCH_SIZE"] * 1024.0 for k, d in f.items() if "FETCH_SIZE" in d}
     kw = nbytes / (w[copy]["WRITE_SIZE"] * 1024.0)
     out = {}
     for k in sorted(set(f) & set(w)):
-        fk, wk = f[k].get("FETCH_SIZE"), w[k].get("WRITE_SIZE")
-        if fk is None or wk is None:
+        fb, wk = fetch.get(k), w[k].get("WRITE_SIZE")
+        if fb is None or wk is None:
             continue
-        out[k] = {"fetch_KiB": fk, "write_KiB": wk, "hbm_bytes_per_launch": kf * fk * 1024.0 + kw * wk * 1024.0}
+        out[k] = {"fetch_KiB": fb / 1024.0, "write_KiB": wk, "hbm_bytes_per_launch": fb + kw * wk * 1024.0}
     # one RK4 step = stage launches 1..4 with Delta^k at the constant coefficient (STAGE 0 is the
     # right-hand-side evaluation): mvec_kernel<STAGE, HVM> (FP32 only, HVM 1), mtile2_kernel<T,
     # STAGE, HV, TPW> or stage2_kernel<T, S, NTR, STAGE, HV, U>, the first family found per precision
@@ -117,8 +146,11 @@ def traffic(fetch_dir, write_dir, copy_mib):
             if len(st) == 4 and key not in out:
                 out[key] = {"hbm_bytes_per_launch": sum(x["hbm_bytes_per_launch"] for x in st) / 4,
                             "kernel": fam + "<" + (prec + ", " if pre else "") + "STAGE, ...>"}
-    return {"calibration": {"copy_bytes_each_way": nbytes, "fetch_factor": kf, "write_factor": kw},
-            "kernels": out}
+    cal = {"copy_bytes_each_way": nbytes, "fetch_factor": kf, "write_factor": kw,
+           "fetch_method": "read requests by size (128/64/32 B)" if by_request else "FETCH_SIZE x fetch_factor"}
+    if by_request:
+        cal["copy_read_bytes_over_moved"] = fetch[copy] / nbytes
+    return {"calibration": cal, "kernels": out}
 
 
 def main(argv):

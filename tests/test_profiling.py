@@ -35,3 +35,20 @@ def test_step_aggregation(monkeypatch):
     assert abs(f32 - expect) < 1e-6
     f64 = d["kernels"]["fp64 step (Delta^4 in every stage)"]["hbm_bytes_per_launch"]
     assert abs(f64 - 2.0 * expect) < 1e-6
+
+
+def test_traffic_from_read_requests(monkeypatch):
+    """With the read requests counted by size the read bytes need no calibration: 128, 64 and 32
+    bytes per request; the copy kernel becomes a check of the method."""
+    fetch = {"stream_copy_kernel": {"TCC_EA0_RDREQ_128B_sum": 16384.0, "TCC_EA0_RDREQ_64B_sum": 0.0,
+                                    "TCC_EA0_RDREQ_32B_sum": 0.0},
+             "mvec_kernel<2, 1>": {"TCC_EA0_RDREQ_128B_sum": 10.0, "TCC_EA0_RDREQ_64B_sum": 3.0,
+                                   "TCC_EA0_RDREQ_32B_sum": 1.0}}
+    write = {"stream_copy_kernel": {"WRITE_SIZE": 2048.0}, "mvec_kernel<2, 1>": {"WRITE_SIZE": 4.0}}
+    monkeypatch.setattr(P, "counter_stats", lambda paths: fetch if paths == ["f"] else write)
+    d = P.traffic("f", "w", 2.0)
+    cal = d["calibration"]
+    assert cal["fetch_factor"] == 1.0 and abs(cal["write_factor"] - 1.0) < 1e-12
+    assert abs(cal["copy_read_bytes_over_moved"] - 1.0) < 1e-12
+    k = d["kernels"]["mvec_kernel<2, 1>"]
+    assert abs(k["hbm_bytes_per_launch"] - (10 * 128 + 3 * 64 + 32 + 4 * 1024)) < 1e-9
