@@ -79,8 +79,9 @@ def tile_flops(idx, S, ntr, hv=True):
 
 
 def load_traffic(kernel_key):
-    """Calibrated HBM bytes per launch of the stage kernel from the committed rocprofv3 counter
-    summary (profiles/stage_kernel_traffic.json), or None."""
+    """Measured HBM bytes per launch of the stage kernel from the committed rocprofv3 counter
+    summary (profiles/stage_kernel_traffic.json: L2 read requests by size plus WRITE_SIZE), or
+    None."""
     path = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")
     try:
         with open(path) as f:
@@ -295,7 +296,7 @@ def main():
                                + ", average of the 4 launches of a step",
                      "bytes_per_launch": b32,
                      "note": "average FP32 stage launch of the f32 leg (HIP events); algorithmic bytes model "
-                             "bench.stage_bytes; traffic = calibrated FETCH_SIZE + WRITE_SIZE per launch",
+                             "bench.stage_bytes; traffic = L2 read requests x size (128/64/32 B) + WRITE_SIZE per launch",
                      "mfma_view": {"dense_flops_per_launch": dense_fl, "useful_flops_per_launch": useful_fl,
                                    "mean_tile_union": umean, "achieved_TFLOPs": mfma_tf,
                                    "peak_TFLOPs": MFMA_F32_PEAK_TFLOPS, "frac": mfma_tf / MFMA_F32_PEAK_TFLOPS}
