@@ -8,8 +8,9 @@
 // of its 16 nodes (and the Delta^k weights when the stage applies hyperviscosity) into dense
 // 16 x U blocks in LDS, multiplies them by the union's columns on the matrix cores, and applies
 // the vertical stencil, the velocity and the fused RK4 update to the result.
-// The kernel (mtile2_kernel, stage-kernel version 5, the default) reads the union columns
-// straight from memory as the B operand, each column once per tile.
+// Two kernels read the union columns straight from memory as the B operand, each column once
+// per tile: mtile2_kernel (stage-kernel version 5: FP64, and FP32 shapes other than 4 tracers
+// of 64 levels) and mvec_kernel (version 7, the FP32 default for 4 tracers of 64 levels, below).
 // MFMA operand layout (16x16x4, lane l, c = l % 16, g = l / 16):
 //   A (16 x 4): A[c][g]     B (4 x 16): B[g][c]
 //   D (16 x 16): FP32 D[4g + i][c], FP64 D[g + 4i][c], i = 0..3
