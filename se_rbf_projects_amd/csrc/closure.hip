@@ -247,8 +247,10 @@ __device__ __forceinline__ float fwd_infer(const InferRegs &r, v4s xb)
 }
 
 // asinh(x) = log(|x| + sqrt(x^2 + 1)) with the sign of x, on the hardware log2 and square root;
-// above 2^12, x^2 + 1 rounds to x^2 and the form log(2|x|) is used.  Its error (a few units of the
-// last place of log) is far below the BF16 rounding of the normalised feature it feeds.
+// above 2^12, x^2 + 1 rounds to x^2 and the form log(2|x|) is used.  Its error is a few units of
+// the last place of the result, except near 0, where forming 1 + |x| costs relative precision
+// but the absolute error stays below about 1e-7: either is far below the BF16 rounding of the
+// normalised feature it feeds.
 __device__ __forceinline__ float asinh_fast(float x)
 {
     const float ax = fabsf(x);
@@ -691,7 +693,9 @@ cvec_kernel(FeatArgs<float> a, NetView nv, MTilePlan tp, int up, const float *nm
     if (qm > 0.f && qm <= 3.0e38f) {
         int E;
         (void)frexpf(qm, &E);
-        sig = ldexpf(1.f, -E);
+        // |E| <= 100 keeps sig and the K / sig that undoes it finite (maxima beyond 2^100 then
+        // saturate the FP16 products, below 2^-100 they lose bits: far outside any tracer field)
+        sig = ldexpf(1.f, -(E < -100 ? -100 : (E > 100 ? 100 : E)));
     }
     __syncthreads();
 
