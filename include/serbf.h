@@ -164,9 +164,10 @@ int serbf_set_stencil_sizes(serbf_engine *e, const int *nn, const int *deg);
 /* Autotuner of the batched weight solves: times each schedule variant that fits on `sample`
  * evenly spaced nodes and keeps the fastest for serbf_weights.  Variants 0-3: the LDS kernel
  * (one wavefront per system, rows in LDS) with the refinement residual re-evaluated (0, 1) or
- * read from an LDS copy of the matrix (2, 3); 4-5 (4 the default): the register-row kernel
- * (one wavefront per system, one or two rows per lane in registers, one persistent launch per
- * size class and system kind); even variants in node order, odd ones largest systems first.
+ * read from an LDS copy of the matrix (2, 3); 4-5 (4 the default): rows in registers (one or
+ * two wavefronts per system with one row per thread up to 96 unknowns, four wavefronts with half
+ * a row per thread up to 112; one persistent launch per size class and system kind); even
+ * variants in node order, odd ones largest systems first.
  * report[3*v + {0,1,2}] = variant, ms (-1: does not fit), systems per second; *nvariants = 6
  * (report holds at least 48 doubles). */
 int serbf_weights_autotune(serbf_engine *e, int sample, double *report, int *nvariants);

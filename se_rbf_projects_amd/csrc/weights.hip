@@ -333,6 +333,14 @@ int row_class_of(int m)
     return -1;
 }
 
+// register schedules of a system of m unknowns: the register-row classes, then the half-row
+// class (rowhalf.hip) up to 112 unknowns (index kNRowClasses); -1 none
+int reg_class_of(int m)
+{
+    const int ci = row_class_of(m);
+    return ci >= 0 ? ci : (m <= 112 ? kNRowClasses : -1);
+}
+
 template <int MAXM>
 int class_bytes(int copy)
 {
@@ -438,9 +446,9 @@ int hv_size(const serbf_engine *e, int i, int m_grad)
 // LDS kernel: the items are grouped by LDS class (the smallest class that holds the node's
 // systems), and each non-empty group is one persistent launch of its class, largest class
 // first, so uneven stencils run their small systems at the occupancy of a small class.
-// Register-row kernel: the gradient systems grouped by register class, then the Delta^k
-// systems likewise, one persistent launch per non-empty group; nodes with a system larger than
-// the largest register class take the LDS kernel for both.
+// Register-row kernel: the gradient systems grouped by register class (the systems of 97..112
+// unknowns by the half-row kernel), then the Delta^k systems likewise, one persistent launch per
+// non-empty group.
 int run_variant(serbf_engine *e, int v, const std::vector<int> &m, const std::vector<int> &items, float *ms)
 {
     const bool hv = e->cfg.hv_order > 0;
@@ -453,18 +461,18 @@ int run_variant(serbf_engine *e, int v, const std::vector<int> &m, const std::ve
     std::vector<int> lds_items;
     if (kWVariants[v].rows) {
         for (int kind = 0; kind < (hv ? 2 : 1); kind++)
-            for (int ci = kNRowClasses - 1; ci >= 0; ci--) {
+            for (int ci = kNRowClasses; ci >= 0; ci--) {
                 const int off = (int)grouped.size();
                 for (int it : items) {
                     const int m1 = m[it], m2 = hv ? hv_size(e, it, m1) : 0;
-                    if (row_class_of(m1) < 0 || row_class_of(m2) < 0) continue;
-                    if (row_class_of(kind == 0 ? m1 : m2) == ci) grouped.push_back(it);
+                    if (reg_class_of(m1) < 0 || reg_class_of(m2) < 0) continue;
+                    if (reg_class_of(kind == 0 ? m1 : m2) == ci) grouped.push_back(it);
                 }
                 if ((int)grouped.size() > off) plan.push_back({off, (int)grouped.size() - off, 1, ci, kind});
             }
         for (int it : items) {
             const int m1 = m[it], m2 = hv ? hv_size(e, it, m1) : 0;
-            if (row_class_of(m1) < 0 || row_class_of(m2) < 0) lds_items.push_back(it);
+            if (reg_class_of(m1) < 0 || reg_class_of(m2) < 0) lds_items.push_back(it);
         }
     } else {
         lds_items = items;
@@ -486,7 +494,7 @@ int run_variant(serbf_engine *e, int v, const std::vector<int> &m, const std::ve
         a.count = L.cnt;
         SERBF_HIP(hipMemsetAsync(e->d_scratch, 0, sizeof(int), e->stream));
         if (L.engine == 1) {
-            rc = launch_row_class(e, a, L.cls, L.kind);
+            rc = L.cls == kNRowClasses ? launch_rowhalf(e, a, L.kind) : launch_row_class(e, a, L.cls, L.kind);
         } else {
             const int copy = kWVariants[v].copy && variant_fits(kClasses[L.cls], 1) ? 1 : 0;
             rc = launch_class(e, a, kClasses[L.cls], copy);

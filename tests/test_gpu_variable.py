@@ -64,6 +64,34 @@ def test_uneven_weights_match_oracle(var):
         assert np.all(Dx[p, var["nn"][p]:] == 0) and np.all(L[p, var["nn"][p]:] == 0)
 
 
+def test_register_and_lds_schedules_match_oracle(var):
+    """The register schedule (variant 4: register-row classes up to 96 unknowns, the half-row
+    kernel above) and the LDS schedule (variant 0) on the same uneven systems, among them gradient
+    systems of 97..105 unknowns (n_i up to 84 with degree-5 polynomials): both against the
+    oracle's gradient weights."""
+    eng, nn, deg = var["eng"], var["nn"], var["deg"]
+    m1 = nn + (deg + 1) * (deg + 2) // 2
+    assert (m1 > 96).sum() > 0 and m1.max() <= 112
+    oDx, oDy, _, ost = O.weights_var(var["cfg"], var["xyz"], var["idx"], nn, deg)
+    assert not ost.any()
+    big = np.nonzero(m1 > 96)[0]
+    try:
+        for v in (4, 0):
+            eng.weights_variant(v)
+            eng.weights()
+            assert eng.weights_stats()[2] == v
+            Dx, Dy, _, st = eng.get_weights()
+            assert not st.any()
+            for a, b in ((Dx, oDx), (Dy, oDy)):
+                err = np.max(np.abs(a - b), axis=1) / np.max(np.abs(b), axis=1)
+                print(f"variant {v}: max rel. weight difference {err.max():.2e}, "
+                      f"over the {len(big)} systems above 96 unknowns {err[big].max():.2e}")
+                assert err.max() < 1e-9
+    finally:
+        eng.weights_variant(var["stats"][2])
+        eng.weights()
+
+
 def _saddle(xyz, idx_row, n, d, phs=7):
     """The config-mode gradient system of one node, assembled in numpy: rotate to the node's
     tangent frame (pole fixed), shift to the centre, scale by the stencil radius."""
