@@ -211,7 +211,11 @@ __device__ __forceinline__ void hr_load_block(double (&row)[kHN], int cb, const 
 
 // kind 0: the gradient systems; kind 1: the Delta^k systems.  Every work item's system has at
 // most 112 unknowns and the back substitution's hand-over needs m > 64.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1)))
+// 2 wavefronts per SIMD (two systems per CU): the 256 registers then spill about 380 bytes per
+// lane to scratch, and still the config-C solves take 298 ms against 348 ms at one wavefront per
+// SIMD without spills (every step's barriers and broadcasts wait less with a second system to
+// switch to); the weights are bit for bit the same.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 rowhalf_kernel(WArgs a, int kind)
 {
     __shared__ __align__(16) double X[3 * kHMaxA];   // stencil coordinates in the solve frame
