@@ -76,6 +76,16 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 __device__ __forceinline__ float softplus_f(float o) { return o > 20.f ? o : log1pf(expf(o)); }
+// true when a point's features hold a NaN or an infinity: the network would not pass them on
+// (ReLU maps NaN to 0 or keeps it by its sign bit) but return an arbitrary finite output, so
+// such a point takes the fallback s = 1
+__device__ __forceinline__ bool bad_features(const float (&f)[SERBF_CLOSURE_FEATURES])
+{
+    float sum = 0.f;
+#pragma unroll
+    for (int k = 0; k < SERBF_CLOSURE_FEATURES; k++) sum += fabsf(f[k]);
+    return !(sum <= 3.0e38f);
+}
 __device__ __forceinline__ float sigmoid_f(float o) { return 1.f / (1.f + expf(-o)); }
 // hidden-layer activation: ReLU as a signed-integer maximum of the bits, one full-rate
 // instruction (fmaxf(x, 0) takes two: IEEE mode makes it quiet a signalling NaN first).  Negative
@@ -481,6 +491,7 @@ apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef, floa
         fc[2] = (float)(speed / (T)a.umax);
         fc[3] = (float)((T(vv) + T(0.5)) / T(s.Nv));
         const int vs0 = s.vstart[vv];
+        bool bad[TRC];
 #pragma unroll
         for (int t = 0; t < TRC; t++) {
             const T *own = s.qin + ((size_t)p * NT + trc[t]) * S;
@@ -500,6 +511,7 @@ apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef, floa
             f[7] = fc[3];
             f[8] = (float)own[vv];
             f[9] = (float)asinh(K * (T)a.dh * fabs(dz));
+            bad[t] = bad_features(f);
             float xn[16];
 #pragma unroll
             for (int k = 0; k < 16; k++) xn[k] = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
@@ -530,7 +542,7 @@ apply_kernel(FeatArgs<T> a, NetView nv, T *gamma, _Float16 *s16, T hv_coef, floa
         for (int t = 0; t < TRC; t++) {
             if (tr0 + t >= NT || !pok) continue;
             float sv = softplus_f(om[t]);
-            if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
+            if (bad[t] || !(sv >= 0.f)) sv = 1.f;   // non-finite features or output
             if (sv > s_max) sv = s_max;
             if (a.sbound && sv > a.sbound[p]) sv = a.sbound[p];
             if (v >= s.Nv) sv = 0.f;
@@ -810,6 +822,7 @@ cvec_kernel(FeatArgs<float> a, NetView nv, MTilePlan tp, int up, const float *nm
             f[7] = ((float)vv + 0.5f) * inv;
             f[8] = ov[lb];
             f[9] = asinh_fast(K * a.dh * fabsf(dz[i][lb]));
+            const bool bad = bad_features(f);
             float xn[16];
 #pragma unroll
             for (int k = 0; k < 16; k++) xn[k] = k < kF ? (f[k] - nv.norm[k]) * nv.norm[16 + k] : 0.f;
@@ -825,7 +838,7 @@ cvec_kernel(FeatArgs<float> a, NetView nv, MTilePlan tp, int up, const float *nm
                 if (g == st) om = o;
             }
             float sv = softplus_fast(om);
-            if (!(sv >= 0.f)) sv = 1.f;   // NaN (softplus is never negative)
+            if (bad || !(sv >= 0.f)) sv = 1.f;   // non-finite features or output
             if (sv > s_max) sv = s_max;
             if (a.sbound && sv > a.sbound[p]) sv = a.sbound[p];
             if (v >= s.Nv) sv = 0.f;

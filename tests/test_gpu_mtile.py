@@ -271,3 +271,45 @@ def test_closure_tile_evaluation_matches_oracle(deform60, scale):
     assert np.mean(rel < 4e-3) > 0.99 and rel.max() < 2e-2
     # FP32 and FP64 feature sums can flip a BF16 rounding of the network input at a few points
     assert np.mean(rel64 < 4e-3) > 0.99 and rel64.max() < 0.1
+
+
+def test_closure_tile_evaluation_non_finite_state(deform60):
+    """A NaN at one level of one node's tracer-0 column: every multiplier stays finite; tracer 0
+    falls back to the constant coefficient (s = 1) at that level of every node whose stencil holds
+    that node (the levels are separate columns of the products; on the tile plan the NaN reaches
+    every row of the tiles whose union holds it, since 0 x NaN is NaN, and the 12-level vertical
+    window of the node itself) and every point it changes is at s = 1 (the network alone would
+    drop the NaN in a ReLU and return an arbitrary finite value); tracers 1..3 are bit for bit
+    unchanged (their products, features and scales never see tracer 0)."""
+    d = deform60
+    eng, cfg, dt = d["eng"], d["cfg"], d["dt"]
+    q0 = d["q0"]
+    params, norm, rep = CL.load()
+    cl = E.Closure(0)
+    cl.set_params(params, norm)
+    t = 0.3 * cfg.T
+    p = 5000
+    try:
+        eng.set_state(q0)
+        eng.attach_closure(cl, every=16, s_max=rep["s_max"])
+        bound = eng.closure_bound(rep["kappa"])
+        ref = eng.closure_gamma(F32, t, dt) / cfg.hv_coef
+        qn = q0.copy()
+        qn[p, 0, 10] = np.nan
+        eng.set_state(qn)
+        got = eng.closure_gamma(F32, t, dt) / cfg.hv_coef
+        idx, _ = eng.stencils()
+    finally:
+        eng.attach_closure(None)
+        cl.close()
+        eng.set_state(q0)
+    assert np.all(np.isfinite(got))
+    assert np.array_equal(got[:, 1:], ref[:, 1:])
+    fallback = np.minimum(1.0, bound.astype(np.float64))
+    hit = np.nonzero((idx == p).any(axis=1))[0]
+    assert len(hit) > 0
+    assert np.allclose(got[hit, 0, 10], fallback[hit], rtol=1e-6, atol=0)
+    cn, cv = np.nonzero(got[:, 0] != ref[:, 0])
+    assert np.allclose(got[cn, 0, cv], fallback[cn], rtol=1e-6, atol=0)
+    print(f"NaN at node {p}, level 10: {len(hit)} stencils hold it; {len(cn)} points fell back to s = 1, "
+          f"levels {sorted(set(cv.tolist()))}")
