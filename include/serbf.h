@@ -15,6 +15,8 @@
 #ifndef SERBF_H
 #define SERBF_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -226,11 +228,10 @@ int serbf_get_lap_weights(serbf_engine *e, double *L1, double *L2);
  * coefficient, c = hv_coef * s, with the MLP 10 -> 64 -> 64 -> 1 (ReLU hidden layers)
  * evaluated in BF16 on the matrix cores with FP32 accumulation.  s = 1 is the
  * constant-coefficient method, which is also the fallback wherever the features or s are not
- * finite.  s_max
- * and the optional per-node stability bound b_p keep c inside the stability region of the
- * explicit RK4 step: b_p = kappa G_ref / G_p with G_p = sum_j |L_pj| the row sum of the node's
- * Delta^k weights (a Gershgorin radius of the hyperviscosity operator, which scales as h^-8)
- * and G_ref its 99.9th percentile over the nodes.
+ * finite.  s_max and the optional per-node stability bound b_p keep c inside the stability
+ * region of the explicit RK4 step: b_p = kappa G_ref / G_p with G_p = sum_j |L_pj| the row sum
+ * of the node's Delta^k weights (a Gershgorin radius of the hyperviscosity operator, which
+ * scales as h^-8) and G_ref its 99.9th percentile over the nodes.
  *
  * Features of a point (node p, tracer, level v), with h the node's nearest-neighbour chord
  * (unit sphere) and asinh compression a(y) = asinh(8 y):

@@ -118,3 +118,14 @@ def test_initial_conditions_match_oracle(golden, kind):
     q = E.initial_condition(kind, cfg, xyz, seed=9)
     oq = O.initial(kind, cfg, xyz, seed=9)
     assert np.max(np.abs(q - oq)) < 1e-14
+
+
+def test_public_headers_compile_standalone():
+    """Each public header compiles on its own, as C11 with warnings as errors and as C++17:
+    it includes what its declarations use (size_t)."""
+    for h in ("serbf.h", "serbf_ref.h"):
+        path = os.path.join(ROOT, "include", h)
+        for cmd in (["gcc", "-fsyntax-only", "-std=c11", "-Wall", "-Werror", "-x", "c", path],
+                    ["g++", "-fsyntax-only", "-std=c++17", "-Wall", "-Werror", "-x", "c++", path]):
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            assert r.returncode == 0, (h, r.stderr[-2000:])
