@@ -374,7 +374,8 @@ __global__ void __launch_bounds__(256) features_kernel(FeatArgs<T> a, float *fea
 }
 
 // closure coefficients: gamma[(p*ntr + tr)*S + v] = hv_coef * s, s = softplus(MLP(x)), with
-// s clamped to s_max, s = 1 where it is not finite; levels >= Nv get 0.
+// s clamped to s_max (and the node's bound), s = 1 where the features or s are not finite;
+// levels >= Nv get 0.
 //
 // A wavefront pass covers NPW = 64/S nodes (lane = node slot x level) and a chunk of up to
 // four tracers, i.e. 256 points.  The pass stages the nodes' stencil rows (indices and the

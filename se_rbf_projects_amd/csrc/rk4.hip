@@ -12,9 +12,10 @@
 //     stage 3: F  = F + 2 f;    q_B = q_T + dt f
 //     stage 4: q_T = q_T + dt/6 (F + 1.0 f)
 //
-// Two forms of the stage: the matrix-core tiles of mtile.hip (version 5, the default where
-// its shape applies: config-mode flows, 1 or 4 tracers of 64 levels) and the column-vector
-// kernel below (version 2), which serves every shape.  The velocity is evaluated in-kernel
+// Three forms of the stage: the matrix-core tiles of mtile.hip where their shape applies
+// (config-mode flows, 1 or 4 tracers of 64 levels): the FP32 vector tiles (version 7, the
+// default, at 4 tracers) and the tiles of version 5 (FP64 and the other FP32 shapes); and the
+// column-vector kernel below (version 2), which serves every shape.  The velocity is evaluated in-kernel
 // from a per-node basis and per-level tables (no velocity arrays are streamed).
 // Work-groups are remapped so that each XCD processes a contiguous range of nodes (its L2
 // sees the spatially local neighbour columns).
