@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libserbf.so")
-SOURCES = ["engine.hip", "rk4.hip", "knn.hip", "weights.hip", "rowlu_96.hip", "rowlu_80.hip", "rowlu_48_64.hip", "rowhalf.hip", "nodes.cpp",
+SOURCES = ["engine.hip", "rk4.hip", "knn.hip", "weights.hip", "rowlu_96.hip", "rowlu_88.hip", "rowlu_80.hip", "rowlu_72.hip", "rowlu_48_64.hip", "rowhalf.hip", "nodes.cpp",
            "nodefile.cpp", "dropin.hip", "tile.hip", "closure.hip", "mtile.hip"]
 ARCH = "gfx950"
 

@@ -434,8 +434,11 @@ int launch_rowlu_t(serbf_engine *e, const WArgs &a, int kind)
 }
 
 int launch_rowlu_48(serbf_engine *e, const WArgs &a, int kind);
+int launch_rowlu_56(serbf_engine *e, const WArgs &a, int kind);
 int launch_rowlu_64(serbf_engine *e, const WArgs &a, int kind);
+int launch_rowlu_72(serbf_engine *e, const WArgs &a, int kind);
 int launch_rowlu_80(serbf_engine *e, const WArgs &a, int kind);
+int launch_rowlu_88(serbf_engine *e, const WArgs &a, int kind);
 int launch_rowlu_96(serbf_engine *e, const WArgs &a, int kind);
 // half-row kernel (rowhalf.hip): systems of 65 .. 112 unknowns, two threads per row
 int launch_rowhalf(serbf_engine *e, const WArgs &a, int kind);

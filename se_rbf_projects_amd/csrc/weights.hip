@@ -322,9 +322,10 @@ constexpr WVariant kWVariants[kNVariants] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0
 
 // register-row classes: MAXA (columns of A held per row), one wavefront up to 64 rows and two
 // above; a row of 96 + 4 doubles is the largest that leaves the kernel room in the 256
-// addressable vector registers
-constexpr int kNRowClasses = 4;
-constexpr int kRowClasses[kNRowClasses] = {48, 64, 80, 96};
+// addressable vector registers.  Every 8 columns, so a system carries at most 7 padding
+// columns through its multiply-adds and pivot-row reads.
+constexpr int kNRowClasses = 7;
+constexpr int kRowClasses[kNRowClasses] = {48, 56, 64, 72, 80, 88, 96};
 
 int row_class_of(int m)
 {
@@ -513,8 +514,11 @@ int launch_row_class(serbf_engine *e, const WArgs &a, int ci, int kind)
 {
     switch (ci) {
     case 0: return launch_rowlu_48(e, a, kind);
-    case 1: return launch_rowlu_64(e, a, kind);
-    case 2: return launch_rowlu_80(e, a, kind);
+    case 1: return launch_rowlu_56(e, a, kind);
+    case 2: return launch_rowlu_64(e, a, kind);
+    case 3: return launch_rowlu_72(e, a, kind);
+    case 4: return launch_rowlu_80(e, a, kind);
+    case 5: return launch_rowlu_88(e, a, kind);
     default: return launch_rowlu_96(e, a, kind);
     }
 }
