@@ -1,8 +1,9 @@
 // Half-row batched LU for the weight systems of 97 .. 112 unknowns (the register-row classes of
 // rowlu.h stop at 96: a row of 112 + 4 doubles does not fit a thread's registers).  One
 // work-group of 4 wavefronts per system: thread t holds half h = t / 128 of row r = t % 128 of
-// [A | B] in registers, columns 64 h .. 64 h + 63 (half 0: A's columns 0..63; half 1: A's
-// columns 64..111, then the four right-hand sides).  The elimination follows rowlu.h (partial
+// [A | B] in registers, columns 56 h .. 56 h + 55 (half 0: A's columns 0..55; half 1: A's
+// columns 56..111, then the four right-hand sides: 56 and 60 registers pairs, which spill less
+// than a split at 64 and took config C's half-row systems from 73.6 to 69 ms).  The elimination follows rowlu.h (partial
 // pivoting on the largest |a_ik|, the lower row on ties; rows are not swapped, each keeps the step
 // at which it became the pivot), with the columns of a block of 4 steps in one known half:
 //   - the pivot: the two wavefronts of the half holding column k reduce |a_ik| (DPP steps and a
@@ -10,7 +11,7 @@
 //   - every thread picks the winner; the pivot row's halves post its columns from k on and the
 //     half holding column k posts each unused row's multiplier; one barrier;
 //   - the unused rows update their half's columns past k with register FMAs.
-// The back substitution runs in half 1 for the columns 64.. (where the right-hand sides are), then
+// The back substitution runs in half 1 for the columns 56.. (where the right-hand sides are), then
 // hands the partial sums to half 0 through LDS; the refinement's forward substitution the other
 // way round.  Its residual b - A x re-evaluates A from the coordinates, each half its own
 // columns.  Same systems, pivot rule and outputs as the register-row and LDS kernels.
@@ -21,13 +22,15 @@
 namespace serbf {
 namespace {
 
-constexpr int kHalf = 64;                        // columns of A per half row
+constexpr int kHalf = 56;                        // columns of A per half row (a multiple of 8)
 constexpr int kHMaxA = 112;                      // unknowns of the class
-constexpr int kHA1 = kHMaxA - kHalf;             // A's columns in half 1 (48)
-constexpr int kHC1 = kHA1 + kRhs;                // half 1's columns with the right-hand sides (52)
-constexpr int kHN = kHalf > kHC1 ? kHalf : kHC1; // register row length (64)
+constexpr int kHA1 = kHMaxA - kHalf;             // A's columns in half 1 (56)
+constexpr int kHC1 = kHA1 + kRhs;                // half 1's columns with the right-hand sides (60)
+constexpr int kHN = kHalf > kHC1 ? kHalf : kHC1; // register row length (60)
 constexpr int kHBlocks = kHMaxA / kRowChunk;     // blocks of 4 elimination steps (28)
-constexpr int kHSwitch = kHalf / kRowChunk;      // first block of half 1 (16)
+constexpr int kHSwitch = kHalf / kRowChunk;      // first block of half 1 (14)
+// the assembly loads 8-column blocks of A up to kHN: both halves' A columns must fit them
+static_assert(kHalf % 8 == 0 && kHA1 <= 8 * (kHN / 8), "half-row split");
 
 struct HalfSys {
     double *pr, *lm, *dinv, *red, *redv, *xfer, *rb;
@@ -210,7 +213,7 @@ __device__ __forceinline__ void hr_load_block(double (&row)[kHN], int cb, const 
 }
 
 // kind 0: the gradient systems; kind 1: the Delta^k systems.  Every work item's system has at
-// most 112 unknowns and the back substitution's hand-over needs m > 64.
+// most 112 unknowns and the back substitution's hand-over needs m > 56.
 // 2 wavefronts per SIMD (two systems per CU): the 256 registers then spill about 380 bytes per
 // lane to scratch, and still the config-C solves take 298 ms against 348 ms at one wavefront per
 // SIMD without spills (every step's barriers and broadcasts wait less with a second system to
